@@ -1,0 +1,222 @@
+/*
+ * seticore_amd: MI355X-native Taylor-tree dedoppler search, C ABI.
+ *
+ * This is the drop-in boundary for lacker/seticore's Dedopplerer
+ * (dedoppler.h:15-79, dedoppler.cu:79-320). The reference exposes a C++ class;
+ * every entry point below replaces one of its members, cited per function.
+ * Plain pointers and sizes only: no C++ types, no torch types. All device
+ * pointers are HIP device (or managed) allocations on the current device.
+ *
+ * Errors: every function returns an sc_status. On failure the message for the
+ * calling thread is available from sc_last_error_string(). Nothing in this
+ * library calls exit() or abort().
+ */
+#ifndef SETICORE_AMD_H
+#define SETICORE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SC_ABI_VERSION 1
+
+typedef enum {
+  SC_OK = 0,
+  SC_ERR_INVALID_ARGUMENT = 1, /* a precondition the reference asserts (dedoppler.cu:191-192) */
+  SC_ERR_HIP = 2,              /* a HIP runtime error (the reference exits with code 2) */
+  SC_ERR_OUT_OF_MEMORY = 3,
+  SC_ERR_UNSUPPORTED = 4,      /* a shape this build has no kernel for */
+  SC_ERR_IO = 5                /* output file errors (writers) */
+} sc_status;
+
+/* Storage type of the power samples handed to sc_dd_search. The reference
+ * only has float32 (filterbank_buffer.h:35). The half-width forms hold
+ * normalised power (see sc_dd_normalize) and are accumulated in float32. */
+typedef enum { SC_F32 = 0, SC_F16 = 1, SC_BF16 = 2 } sc_dtype;
+
+/* Where the column-sum statistics and the window merge run.
+ * SC_STATS_HOST_EXACT follows dedoppler.cu:232-319 on the host: the column
+ * sums and top arrays are copied back and the median/trimmed deviation are
+ * computed with the same std::nth_element calls, so SNR values are bit-equal
+ * to the reference. SC_STATS_DEVICE keeps everything on the GPU: exact rank
+ * median by radix select, FP64 trimmed mean/deviation over the sorted ranks,
+ * and a wave-per-window merge; only the per-window candidates come back.
+ * Default: SC_STATS_HOST_EXACT. */
+typedef enum { SC_STATS_HOST_EXACT = 0, SC_STATS_DEVICE = 1 } sc_stats_mode;
+
+/* The metadata fields a search reads (filterbank_metadata.h:15-62 and
+ * dedoppler_hit.cpp:11-23): isCoherentBeam(beam) is
+ * 0 <= beam < num_coherent_beams (filterbank_metadata.cpp:116-118). */
+typedef struct {
+  double fch1;               /* MHz, frequency of channel 0 of the file */
+  double foff;               /* MHz per fine channel */
+  long coarse_channel_size;  /* fine channels per coarse channel */
+  int num_coherent_beams;    /* size of the per-beam source list */
+} sc_meta;
+
+/* One hit: the fields of DedopplerHit (dedoppler_hit.h:11-45), same meaning. */
+typedef struct {
+  double frequency;       /* MHz: fch1 + (coarse_channel*coarse_channel_size + index)*foff */
+  int index;              /* start fine channel within the coarse channel */
+  int drift_steps;        /* drift_block*(T_r-1) + path_offset */
+  double drift_rate;      /* Hz/s: drift_steps * resolution */
+  float snr;              /* (power - median) / std */
+  int coarse_channel;
+  int beam;               /* beam if coherent, else -1 (NO_BEAM) */
+  int num_timesteps;      /* unrounded */
+  float power;            /* the path sum */
+  float incoherent_power; /* filled by sc_dd_add_incoherent_power, else 0 */
+} sc_hit;
+
+typedef struct sc_dd sc_dd;
+
+/* Replaces Dedopplerer::Dedopplerer (dedoppler.cu:79-111). num_timesteps is
+ * unrounded (> 1); the search runs over roundUpToPowerOfTwo(num_timesteps)
+ * rows. Allocates all device scratch once, on the current device. */
+sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz,
+                       double tsamp_s, int has_dc_spike, sc_dd** out);
+
+/* Replaces Dedopplerer::~Dedopplerer (dedoppler.cu:113-124). */
+sc_status sc_dd_destroy(sc_dd* dd);
+
+/* Replaces Dedopplerer::memoryUsage (dedoppler.cu:127-130): device bytes held. */
+sc_status sc_dd_memory_usage(const sc_dd* dd, size_t* bytes);
+
+/* Selects sc_stats_mode for later searches (no counterpart; default host-exact). */
+sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
+
+/* Shape of the fused tree kernel used when rounded_timesteps <= 16:
+ * channels held per lane (4 or 8) and drift blocks per LDS window (2..8).
+ * Returns SC_ERR_UNSUPPORTED for shapes that are not compiled. Results do not
+ * depend on the shape; only speed does. No counterpart in the reference (its
+ * tile shape is fixed, taylor.cu:170-189); used by the autotuner. */
+sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group);
+
+/* Kernel timing for measurement (no counterpart in the reference, which only
+ * times whole runs, taylor_benchmark.cu:22-40): when enabled, each search
+ * records HIP events on its stream around the path-sum kernels (the tree and
+ * top-path tracking over all drift blocks) and accumulates their elapsed time.
+ * Enabling (or disabling) resets the totals. */
+sc_status sc_dd_set_timing(sc_dd* dd, int enable);
+sc_status sc_dd_kernel_time(const sc_dd* dd, double* total_ms, long* searches);
+
+/* Replaces Dedopplerer::search (dedoppler.cu:186-320).
+ * data: device pointer to [rounded_timesteps][row_stride] samples of `dtype`
+ *       (row_stride >= num_channels, in elements; rows beyond the unrounded
+ *       num_timesteps must already be zero, as the reference's loader leaves
+ *       them, h5_reader.cpp:195-200).
+ * The hits are kept in the handle (ascending channel order, dedoppler.cu:275).
+ * Up to `capacity` of them are copied to `hits`; *num_hits receives the total,
+ * which may exceed capacity (then fetch the rest with sc_dd_copy_hits).
+ * stream: a hipStream_t (NULL = default stream). The call returns after the
+ * search is complete, like the reference's blocking copies (dedoppler.cu:232). */
+sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_timesteps,
+                       int num_channels, long row_stride, const sc_meta* meta,
+                       int beam, int coarse_channel, double max_drift,
+                       double min_drift, double snr_threshold, sc_hit* hits,
+                       size_t capacity, size_t* num_hits, void* stream);
+
+/* Copies hits of the last sc_dd_search (no counterpart: the reference appends
+ * to a vector). */
+sc_status sc_dd_copy_hits(const sc_dd* dd, sc_hit* hits, size_t capacity,
+                          size_t* num_hits);
+
+/* Per-channel state of the last search, copied to host arrays of
+ * num_channels elements each (any may be NULL): column sums (dedoppler.cu:207),
+ * top path sums, their drift blocks and path offsets (dedoppler.cu:32-55),
+ * and the median/std the threshold used. No counterpart (the reference keeps
+ * them in private pinned arrays); used to check parity. */
+sc_status sc_dd_copy_state(const sc_dd* dd, float* column_sums, float* top_sums,
+                           int* top_drift_blocks, int* top_path_offsets,
+                           float* median, float* std_dev);
+
+/* Replaces Dedopplerer::addIncoherentPower (dedoppler.cu:139-176): sorts hits
+ * by drift_steps (driftStepsLessThan, dedoppler_hit.cpp:69-71) and fills each
+ * hit's incoherent_power with the path sum of the same path in `data`. */
+sc_status sc_dd_add_incoherent_power(sc_dd* dd, const void* data, int dtype,
+                                     int rounded_timesteps, int num_channels,
+                                     long row_stride, sc_hit* hits, size_t num_hits,
+                                     void* stream);
+
+/* Computes the full [rounded_timesteps][num_channels] path sums of one drift
+ * block, like optimizedTaylorTree (taylor.cu:333-350), into a device buffer of
+ * rounded_timesteps*num_channels floats laid out [path_offset][start_channel].
+ * Entries of invalid paths are unspecified (the reference leaves them stale).
+ * No separate counterpart in the search; exposed for the reference's
+ * differential tree test (taylor_test.cu:8-51). */
+sc_status sc_dd_taylor_tree(sc_dd* dd, const void* data, int dtype,
+                            int rounded_timesteps, int num_channels, long row_stride,
+                            int drift_block, float* out, void* stream);
+
+/* Per-coarse-channel normalisation for the half-width storage forms: computes
+ * the FP64 mean and standard deviation of all T*N samples on the GPU and
+ * writes (x - mean) / std, clamped to +-clamp, as `out_dtype` ([T][N]).
+ * No counterpart in the reference (it only stores float32). */
+sc_status sc_normalize(const float* data, int rounded_timesteps, int num_channels,
+                       long row_stride, int out_dtype, float clamp, void* out,
+                       double* mean_out, double* std_out, void* stream);
+
+/* Synthetic waterfall generator used by the tests and the benchmark (not part
+ * of the reference): exponential(1) noise from a counter-based hash of
+ * (seed, t, c), times `gain`, a cosine taper over the outer `taper_frac` of the
+ * channels, a DC spike of `dc_spike` times the mean at num_channels/2 (0 = off),
+ * plus `num_tones` linear tones. Tone k starts at channel tone_start[k] at t=0,
+ * moves tone_slope[k] channels per time step and adds tone_amp[k] to the nearest
+ * channel of each of the first `num_timesteps` rows. Rows in
+ * [num_timesteps, rounded_timesteps) are zero. Writes float32 [T_r][num_channels]. */
+sc_status sc_gen_waterfall(float* out, int num_timesteps, int rounded_timesteps,
+                           int num_channels, uint64_t seed, float gain,
+                           float taper_frac, float dc_spike, int num_tones,
+                           const float* tone_start, const float* tone_slope,
+                           const float* tone_amp, void* stream);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* sc_last_error_string(void);
+
+/* SC_ABI_VERSION of the built library. */
+int sc_abi_version(void);
+
+/* ---- Output writers (dat_file_writer.cpp, hit_file_writer.cpp) ---------- */
+
+/* Header fields the writers take from the file metadata
+ * (filterbank_metadata.h:15-62). source_name may be NULL ("" is written). */
+typedef struct {
+  const char* source_name;
+  double fch1, foff, tstart, tsamp, src_raj, src_dej;
+  long num_timesteps;        /* unrounded */
+  long coarse_channel_size;
+  int telescope_id;
+} sc_file_meta;
+
+typedef struct sc_writer sc_writer;
+
+/* Replaces DatFileWriter (dat_file_writer.cpp:15-73): turboSETI header at
+ * open, one row per hit. */
+sc_status sc_dat_writer_open(const char* filename, const sc_file_meta* meta,
+                             double max_drift, sc_writer** out);
+
+/* Replaces HitFileWriter (hit_file_writer.cpp:16-97): one unpacked Cap'n Proto
+ * message per hit (schema hit.capnp), written to filename + ".tmp" and renamed
+ * on close. `beam_source_names`/`beam_ras`/`beam_decs` are the per-beam
+ * metadata (num_beams entries, may be 0/NULL). */
+sc_status sc_hits_writer_open(const char* filename, const sc_file_meta* meta,
+                              int num_beams, const char* const* beam_source_names,
+                              const double* beam_ras, const double* beam_decs,
+                              sc_writer** out);
+
+/* Replaces HitRecorder::recordHit (hit_recorder.h:10-19). `input` is the host
+ * copy of the coarse channel's [T][coarse_channel_size] float32 data (read by
+ * the .hits writer for its filterbank slice; may be NULL for .dat). */
+sc_status sc_writer_record_hit(sc_writer* w, const sc_hit* hit, const float* input);
+
+/* Closes the file (and renames .hits.tmp to .hits). */
+sc_status sc_writer_close(sc_writer* w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SETICORE_AMD_H */

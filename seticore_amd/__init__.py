@@ -1,0 +1,499 @@
+"""seticore_amd: MI355X-native Taylor-tree dedoppler search.
+
+Python mirror of lacker/seticore's dedoppler interface, calling the HIP
+kernels through the C ABI in ``include/seticore_amd.h``:
+
+* :class:`Dedopplerer` - dedoppler.h:15-79 (``search``, ``addIncoherentPower``,
+  ``memoryUsage``, ``print_hits``).
+* :class:`FilterbankBuffer` - filterbank_buffer.h:11-48 (``[time][channel]``
+  float32 on the GPU, owning or a view).
+* :class:`FilterbankMetadata` - the fields of filterbank_metadata.h:15-62 that
+  the search and the writers read.
+* :class:`DedopplerHit` - dedoppler_hit.h:11-66.
+* :class:`DatFileWriter`, :class:`HitFileWriter`, :func:`makeHitRecorder` -
+  hit_recorder.h, dat_file_writer.h, hit_file_writer.h.
+
+Device memory and streams come from PyTorch (ROCm); every computation runs in
+``libseticore_amd.so``. There is no CPU fallback: if the library or a GPU is
+missing the calls raise.
+"""
+
+import ctypes
+import math
+import os
+
+__version__ = "0.1.0"
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libseticore_amd.so")
+
+NO_BEAM = -1  # dedoppler_hit.h:9
+
+SC_OK = 0
+SC_ERR_INVALID_ARGUMENT = 1
+SC_ERR_HIP = 2
+SC_ERR_OUT_OF_MEMORY = 3
+SC_ERR_UNSUPPORTED = 4
+SC_ERR_IO = 5
+
+SC_F32, SC_F16, SC_BF16 = 0, 1, 2
+SC_STATS_HOST_EXACT, SC_STATS_DEVICE = 0, 1
+
+
+class SeticoreError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("seticore_amd error %d: %s" % (status, message))
+        self.status = status
+
+
+class ScMeta(ctypes.Structure):
+    _fields_ = [("fch1", ctypes.c_double), ("foff", ctypes.c_double),
+                ("coarse_channel_size", ctypes.c_long),
+                ("num_coherent_beams", ctypes.c_int)]
+
+
+class ScHit(ctypes.Structure):
+    _fields_ = [("frequency", ctypes.c_double), ("index", ctypes.c_int),
+                ("drift_steps", ctypes.c_int), ("drift_rate", ctypes.c_double),
+                ("snr", ctypes.c_float), ("coarse_channel", ctypes.c_int),
+                ("beam", ctypes.c_int), ("num_timesteps", ctypes.c_int),
+                ("power", ctypes.c_float), ("incoherent_power", ctypes.c_float)]
+
+
+class ScFileMeta(ctypes.Structure):
+    _fields_ = [("source_name", ctypes.c_char_p), ("fch1", ctypes.c_double),
+                ("foff", ctypes.c_double), ("tstart", ctypes.c_double),
+                ("tsamp", ctypes.c_double), ("src_raj", ctypes.c_double),
+                ("src_dej", ctypes.c_double), ("num_timesteps", ctypes.c_long),
+                ("coarse_channel_size", ctypes.c_long), ("telescope_id", ctypes.c_int)]
+
+
+_lib_handle = None
+
+
+def lib():
+    """Loads libseticore_amd.so (built in-tree by seticore_amd.build)."""
+    global _lib_handle
+    if _lib_handle is not None:
+        return _lib_handle
+    if not os.path.exists(LIBRARY_PATH):
+        raise SeticoreError(SC_ERR_UNSUPPORTED,
+                            "%s is not built; run python -m seticore_amd.build" % LIBRARY_PATH)
+    L = ctypes.CDLL(LIBRARY_PATH)
+    vp, i, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    lng = ctypes.c_long
+    P = ctypes.POINTER
+    sig = {
+        "sc_dd_create": [i, i, d, d, i, P(vp)],
+        "sc_dd_destroy": [vp],
+        "sc_dd_memory_usage": [vp, P(sz)],
+        "sc_dd_set_stats_mode": [vp, i],
+        "sc_dd_set_tuning": [vp, i, i],
+        "sc_dd_set_timing": [vp, i],
+        "sc_dd_kernel_time": [vp, P(d), P(lng)],
+        "sc_dd_search": [vp, vp, i, i, i, lng, P(ScMeta), i, i, d, d, d, P(ScHit), sz,
+                         P(sz), vp],
+        "sc_dd_copy_hits": [vp, P(ScHit), sz, P(sz)],
+        "sc_dd_copy_state": [vp, vp, vp, vp, vp, P(ctypes.c_float), P(ctypes.c_float)],
+        "sc_dd_add_incoherent_power": [vp, vp, i, i, i, lng, P(ScHit), sz, vp],
+        "sc_dd_taylor_tree": [vp, vp, i, i, i, lng, i, vp, vp],
+        "sc_normalize": [vp, i, i, lng, i, ctypes.c_float, vp, P(d), P(d), vp],
+        "sc_gen_waterfall": [vp, i, i, i, ctypes.c_uint64, ctypes.c_float, ctypes.c_float,
+                             ctypes.c_float, i, P(ctypes.c_float), P(ctypes.c_float),
+                             P(ctypes.c_float), vp],
+        "sc_dat_writer_open": [ctypes.c_char_p, P(ScFileMeta), d, P(vp)],
+        "sc_hits_writer_open": [ctypes.c_char_p, P(ScFileMeta), i, P(ctypes.c_char_p),
+                                P(d), P(d), P(vp)],
+        "sc_writer_record_hit": [vp, P(ScHit), P(ctypes.c_float)],
+        "sc_writer_close": [vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.sc_last_error_string.argtypes = []
+    L.sc_last_error_string.restype = ctypes.c_char_p
+    L.sc_abi_version.argtypes = []
+    L.sc_abi_version.restype = ctypes.c_int
+    _lib_handle = L
+    return L
+
+
+def check(status):
+    if status != SC_OK:
+        msg = lib().sc_last_error_string().decode("utf-8", "replace")
+        raise SeticoreError(status, msg)
+
+
+def round_up_to_power_of_two(n):
+    """util.cpp:16-22"""
+    r = 1
+    while r < n:
+        r *= 2
+    return r
+
+
+roundUpToPowerOfTwo = round_up_to_power_of_two
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+# ---------------------------------------------------------------------------
+class FilterbankMetadata:
+    """The metadata fields a search and the writers read
+    (filterbank_metadata.h:15-62)."""
+
+    def __init__(self, fch1=0.0, foff=0.0, coarse_channel_size=0, source_name="",
+                 source_names=None, ras=None, decs=None, tstart=0.0, tsamp=0.0,
+                 src_raj=0.0, src_dej=0.0, num_timesteps=0, num_channels=0,
+                 telescope_id=-1, has_dc_spike=False):
+        self.fch1 = fch1
+        self.foff = foff
+        self.coarse_channel_size = coarse_channel_size
+        self.source_name = source_name
+        self.source_names = list(source_names or [])
+        self.ras = list(ras or [])
+        self.decs = list(decs or [])
+        self.tstart = tstart
+        self.tsamp = tsamp
+        self.src_raj = src_raj
+        self.src_dej = src_dej
+        self.num_timesteps = num_timesteps
+        self.num_channels = num_channels
+        self.telescope_id = telescope_id
+        self.has_dc_spike = has_dc_spike
+
+    @property
+    def num_coarse_channels(self):
+        return self.num_channels // self.coarse_channel_size if self.coarse_channel_size else 0
+
+    def isCoherentBeam(self, beam):
+        """filterbank_metadata.cpp:116-118"""
+        return 0 <= beam < len(self.source_names)
+
+    def _sc_meta(self):
+        return ScMeta(self.fch1, self.foff, int(self.coarse_channel_size),
+                      len(self.source_names))
+
+    def _sc_file_meta(self):
+        return ScFileMeta(self.source_name.encode(), self.fch1, self.foff, self.tstart,
+                          self.tsamp, self.src_raj, self.src_dej, int(self.num_timesteps),
+                          int(self.coarse_channel_size), int(self.telescope_id))
+
+
+class FilterbankBuffer:
+    """Row-major ``data[time][channel]`` power on the GPU
+    (filterbank_buffer.h:11-48). ``data`` is a torch tensor; pass ``data=`` to
+    make a view on an existing ``[num_timesteps][>= num_channels]`` tensor."""
+
+    def __init__(self, num_timesteps, num_channels, data=None, dtype=SC_F32):
+        torch = _torch()
+        self.num_timesteps = int(num_timesteps)
+        self.num_channels = int(num_channels)
+        self.dtype = dtype
+        tdtype = {SC_F32: torch.float32, SC_F16: torch.float16, SC_BF16: torch.bfloat16}[dtype]
+        if data is None:
+            self.managed = True
+            self.data = torch.zeros((self.num_timesteps, self.num_channels), dtype=tdtype,
+                                    device="cuda")
+        else:
+            self.managed = False
+            if data.dim() != 2 or data.shape[0] != self.num_timesteps or \
+                    data.shape[1] < self.num_channels or data.stride(1) != 1:
+                raise ValueError("view must be [num_timesteps][>= num_channels], unit stride")
+            if data.dtype != tdtype or not data.is_cuda:
+                raise ValueError("view must be a CUDA tensor of the buffer dtype")
+            self.data = data
+        self.size = self.num_timesteps * self.num_channels
+
+    @property
+    def row_stride(self):
+        return self.data.stride(0)
+
+    def zero(self):
+        self.data.zero_()
+
+    def set(self, time, channel, value):
+        assert 0 <= time < self.num_timesteps and 0 <= channel < self.num_channels
+        self.data[time, channel] = value
+
+    def get(self, time, channel):
+        return float(self.data[time, channel].item())
+
+
+def makeNoisyBuffer(num_timesteps, num_channels):
+    """filterbank_buffer.cu:71-78: row 0 = 0.1*chan/num_channels, rest 0."""
+    torch = _torch()
+    buf = FilterbankBuffer(num_timesteps, num_channels)
+    buf.zero()
+    row = torch.tensor([0.1 * c / num_channels for c in range(num_channels)],
+                       dtype=torch.float64).to(torch.float32)
+    buf.data[0, :] = row.to("cuda")
+    return buf
+
+
+class DedopplerHit:
+    """dedoppler_hit.h:11-66"""
+
+    __slots__ = ("frequency", "index", "drift_steps", "drift_rate", "snr", "coarse_channel",
+                 "beam", "num_timesteps", "power", "incoherent_power")
+
+    @classmethod
+    def _from_c(cls, h):
+        o = cls.__new__(cls)
+        for name, _ in ScHit._fields_:
+            setattr(o, name, getattr(h, name))
+        return o
+
+    def _to_c(self):
+        return ScHit(*[getattr(self, name) for name, _ in ScHit._fields_])
+
+    def lowIndex(self):
+        return min(self.index, self.index + self.drift_steps)
+
+    def highIndex(self):
+        return max(self.index, self.index + self.drift_steps)
+
+    def toString(self):
+        """dedoppler_hit.cpp:25-30 (fmt "{:.5f}" == printf "%.5f")."""
+        n = self.drift_steps
+        bins = "%d bin" % n if n in (1, -1) else "%d bins" % n
+        return "coarse channel = %d, index = %d, snr = %.5f, drift rate = %.5f (%s)" % (
+            self.coarse_channel, self.index, self.snr, self.drift_rate, bins)
+
+    def __repr__(self):
+        return "DedopplerHit(%s)" % self.toString()
+
+    def key(self):
+        return (self.index, self.drift_steps, self.coarse_channel, self.beam)
+
+
+class Dedopplerer:
+    """dedoppler.h:15-79, running on the GPU through libseticore_amd.so."""
+
+    def __init__(self, num_timesteps, num_channels, foff, tsamp, has_dc_spike=False):
+        L = lib()
+        self.num_timesteps = int(num_timesteps)
+        self.num_channels = int(num_channels)
+        self.foff = float(foff)
+        self.tsamp = float(tsamp)
+        self.has_dc_spike = bool(has_dc_spike)
+        self.print_hits = False
+        self.rounded_num_timesteps = round_up_to_power_of_two(self.num_timesteps)
+        h = ctypes.c_void_p()
+        _torch().cuda.init()
+        check(L.sc_dd_create(self.num_timesteps, self.num_channels, self.foff, self.tsamp,
+                             int(self.has_dc_spike), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sc_dd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def memoryUsage(self):
+        n = ctypes.c_size_t()
+        check(lib().sc_dd_memory_usage(self._h, ctypes.byref(n)))
+        return n.value
+
+    def set_stats_mode(self, mode):
+        check(lib().sc_dd_set_stats_mode(self._h, int(mode)))
+
+    def set_tuning(self, channels_per_lane, drift_group):
+        check(lib().sc_dd_set_tuning(self._h, int(channels_per_lane), int(drift_group)))
+
+    def set_timing(self, enable):
+        check(lib().sc_dd_set_timing(self._h, int(bool(enable))))
+
+    def kernel_time(self):
+        """(total ms of the path-sum kernels, number of timed searches)."""
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        check(lib().sc_dd_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def search(self, input, metadata, beam, coarse_channel, max_drift, min_drift,
+               snr_threshold, output, stream=None):
+        """dedoppler.cu:186-320: appends this coarse channel's hits to `output`."""
+        L = lib()
+        meta = metadata._sc_meta()
+        cap = 256
+        hits = (ScHit * cap)()
+        n = ctypes.c_size_t()
+        check(L.sc_dd_search(self._h, ctypes.c_void_p(input.data.data_ptr()), input.dtype,
+                             input.num_timesteps, input.num_channels, input.row_stride,
+                             ctypes.byref(meta), int(beam), int(coarse_channel),
+                             float(max_drift), float(min_drift), float(snr_threshold), hits,
+                             cap, ctypes.byref(n), _stream_ptr(stream)))
+        if n.value > cap:
+            hits = (ScHit * n.value)()
+            check(L.sc_dd_copy_hits(self._h, hits, n.value, ctypes.byref(n)))
+        new = [DedopplerHit._from_c(hits[k]) for k in range(n.value)]
+        if self.print_hits:
+            for h in new:
+                print("hit: " + h.toString())
+        output.extend(new)
+        return new
+
+    def state(self):
+        """Column sums, top sums/blocks/offsets, median and std of the last search."""
+        import numpy as np
+        N = self.num_channels
+        cs = np.empty(N, np.float32)
+        ts = np.empty(N, np.float32)
+        tb = np.empty(N, np.int32)
+        to = np.empty(N, np.int32)
+        med = ctypes.c_float()
+        sd = ctypes.c_float()
+        check(lib().sc_dd_copy_state(self._h, cs.ctypes.data, ts.ctypes.data, tb.ctypes.data,
+                                     to.ctypes.data, ctypes.byref(med), ctypes.byref(sd)))
+        return {"column_sums": cs, "top_sums": ts, "top_blocks": tb, "top_offsets": to,
+                "median": med.value, "std": sd.value}
+
+    def addIncoherentPower(self, input, hits, stream=None):
+        """dedoppler.cu:139-176: sorts `hits` by drift steps, fills incoherent_power."""
+        arr = (ScHit * max(1, len(hits)))(*[h._to_c() for h in hits])
+        check(lib().sc_dd_add_incoherent_power(
+            self._h, ctypes.c_void_p(input.data.data_ptr()), input.dtype, input.num_timesteps,
+            input.num_channels, input.row_stride, arr, len(hits), _stream_ptr(stream)))
+        hits[:] = [DedopplerHit._from_c(arr[k]) for k in range(len(hits))]
+
+    def taylor_tree(self, input, drift_block, stream=None):
+        """[T_r][N] path sums of one drift block (optimizedTaylorTree, taylor.cu:333-350)."""
+        torch = _torch()
+        out = torch.empty((self.rounded_num_timesteps, self.num_channels), dtype=torch.float32,
+                          device="cuda")
+        check(lib().sc_dd_taylor_tree(self._h, ctypes.c_void_p(input.data.data_ptr()),
+                                      input.dtype, input.num_timesteps, input.num_channels,
+                                      input.row_stride, int(drift_block),
+                                      ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)))
+        return out
+
+
+def drift_parameters(num_timesteps, foff, tsamp, max_drift):
+    """dedoppler.cu:87,195-198,274: (min_block, max_block, window, resolution)."""
+    T = round_up_to_power_of_two(num_timesteps)
+    res = 1e6 * foff / ((T - 1) * tsamp)
+    nmd = max_drift / abs(res * (T - 1))
+    return (int(math.floor(-nmd)), int(math.floor(nmd)), int(2 * math.ceil(nmd * (T - 1))), res)
+
+
+# ---------------------------------------------------------------------------
+class _Writer:
+    def __init__(self, handle):
+        self._h = handle
+
+    def recordHit(self, hit, input=None):
+        """hit_recorder.h:10-19. `input` is the host float32 [T][N] coarse channel
+        (numpy array or torch tensor) - the .hits writer copies a slice of it."""
+        ptr = None
+        keep = None
+        if input is not None:
+            import numpy as np
+            if hasattr(input, "detach"):
+                input = input.detach().float().cpu().numpy()
+            keep = np.ascontiguousarray(input, dtype=np.float32)
+            ptr = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        c = hit._to_c()
+        check(lib().sc_writer_record_hit(self._h, ctypes.byref(c), ptr))
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, None
+            check(lib().sc_writer_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DatFileWriter(_Writer):
+    """dat_file_writer.h:15-27"""
+
+    def __init__(self, filename, metadata, max_drift):
+        h = ctypes.c_void_p()
+        fm = metadata._sc_file_meta()
+        check(lib().sc_dat_writer_open(filename.encode(), ctypes.byref(fm), float(max_drift),
+                                       ctypes.byref(h)))
+        super().__init__(h)
+
+
+class HitFileWriter(_Writer):
+    """hit_file_writer.h:16-31"""
+
+    def __init__(self, filename, metadata):
+        h = ctypes.c_void_p()
+        fm = metadata._sc_file_meta()
+        nb = len(metadata.source_names)
+        names = (ctypes.c_char_p * max(1, nb))(*[s.encode() for s in metadata.source_names])
+        ras = (ctypes.c_double * max(1, nb))(*metadata.ras)
+        decs = (ctypes.c_double * max(1, nb))(*metadata.decs)
+        check(lib().sc_hits_writer_open(filename.encode(), ctypes.byref(fm), nb, names, ras,
+                                        decs, ctypes.byref(h)))
+        super().__init__(h)
+
+
+def makeHitRecorder(filename, metadata, max_drift):
+    """hit_recorder.cpp:10-23: picks the writer by suffix."""
+    if filename.endswith(".dat"):
+        return DatFileWriter(filename, metadata, max_drift)
+    if filename.endswith(".hits"):
+        return HitFileWriter(filename, metadata)
+    raise SeticoreError(SC_ERR_INVALID_ARGUMENT,
+                        "could not recognize the file type of %s" % filename)
+
+
+def gen_waterfall(out, num_timesteps, seed, gain=1.0, taper_frac=0.0, dc_spike=0.0,
+                  tones=(), stream=None):
+    """Fills a float32 CUDA tensor [T_r][N] with seeded synthetic power:
+    exponential(1) noise * gain (cosine taper on the outer taper_frac of the
+    channels, optional DC spike) plus linear tones (start, slope, amplitude)."""
+    T_r, N = out.shape
+    n = len(tones)
+    F = ctypes.c_float * max(1, n)
+    st = F(*[t[0] for t in tones])
+    sl = F(*[t[1] for t in tones])
+    am = F(*[t[2] for t in tones])
+    check(lib().sc_gen_waterfall(ctypes.c_void_p(out.data_ptr()), int(num_timesteps), int(T_r),
+                                 int(N), ctypes.c_uint64(seed), float(gain), float(taper_frac),
+                                 float(dc_spike), n, st, sl, am, _stream_ptr(stream)))
+    return out
+
+
+def normalize(data, out_dtype, clamp=65504.0, stream=None):
+    """Per-coarse-channel normalisation (x - mean) / std with FP64 moments on the
+    GPU, stored as `out_dtype`. Returns (tensor, mean, std)."""
+    torch = _torch()
+    T, N = data.shape
+    tdtype = {SC_F32: torch.float32, SC_F16: torch.float16, SC_BF16: torch.bfloat16}[out_dtype]
+    out = torch.empty((T, N), dtype=tdtype, device=data.device)
+    mean = ctypes.c_double()
+    sd = ctypes.c_double()
+    check(lib().sc_normalize(ctypes.c_void_p(data.data_ptr()), int(T), int(N),
+                             int(data.stride(0)), int(out_dtype), float(clamp),
+                             ctypes.c_void_p(out.data_ptr()), ctypes.byref(mean),
+                             ctypes.byref(sd), _stream_ptr(stream)))
+    return out, mean.value, sd.value

@@ -1,0 +1,694 @@
+// C-ABI implementation of the dedoppler search (host side).
+//
+// Replaces lacker/seticore's Dedopplerer (dedoppler.h:15-79, dedoppler.cu:79-320).
+// The orchestration differs from the reference: the whole drift range is one
+// fused launch for T_r <= 16 (the reference launches a tree and a top-path
+// kernel per drift block), and the statistics/merge can stay on the GPU. The
+// host-exact statistics reproduce dedoppler.cu:241-319 step for step.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/seticore_amd.h"
+#include "dd_internal.h"
+#include "sc_error.h"
+
+namespace sc {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+}  // namespace sc
+
+namespace {
+
+sc_status fail(sc_status code, const std::string& msg) {
+  sc::set_last_error(msg);
+  return code;
+}
+
+sc_status hip_fail(hipError_t e, const char* where) {
+  return fail(SC_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define SC_HIP(call, where)                            \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return hip_fail(_e, where);  \
+  } while (0)
+
+// util.cpp:16-22
+int round_up_pow2(int n) {
+  int r = 1;
+  while (r < n) r *= 2;
+  return r;
+}
+
+size_t dtype_size(int dtype) { return dtype == SC_F32 ? 4 : 2; }
+
+// Default fused-kernel shape per T_r: channels per lane and drift blocks per
+// LDS window.
+void default_tuning(int T, int* lanes, int* group) {
+  *lanes = (T == 16) ? 4 : 8;
+  *group = (T == 16) ? 4 : 8;
+}
+
+}  // namespace
+
+struct sc_dd {
+  int num_timesteps = 0;
+  int rounded = 0;
+  int num_channels = 0;
+  double foff = 0, tsamp = 0;
+  bool has_dc_spike = false;
+  int drift_timesteps = 0;
+  double resolution = 0;
+  int stats_mode = SC_STATS_HOST_EXACT;
+  int lanes = 4;  // fused kernel: channels per lane
+  int group = 4;  // fused kernel: drift blocks per LDS window
+  int device = 0;
+
+  // Device state.
+  sc::top_key_t* top_key = nullptr;
+  float* colsum = nullptr;
+  float* top_sum = nullptr;
+  int* top_blk = nullptr;
+  int* top_off = nullptr;
+  void* stats_scratch = nullptr;
+  float* stats_out = nullptr;  // {median, std}
+  int4* cand = nullptr;        // per-window candidates
+  size_t cand_capacity = 0;
+  float* tree_a = nullptr;     // T_r*N scratch for the staged path and tree API
+  float* tree_b = nullptr;
+  int* gather_idx = nullptr;   // 2*capacity ints
+  float* gather_out = nullptr;
+  size_t gather_capacity = 0;
+  size_t device_bytes = 0;
+
+  // Pinned host mirrors.
+  float* h_colsum = nullptr;
+  float* h_top_sum = nullptr;
+  int* h_top_blk = nullptr;
+  int* h_top_off = nullptr;
+  int4* h_cand = nullptr;
+  float* h_stats = nullptr;
+
+  // Optional kernel timing (HIP events on the search stream).
+  bool timing = false;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  double kernel_ms = 0.0;
+  long kernel_launches = 0;
+
+  float last_median = 0, last_std = 0;
+  bool state_valid = false;
+  std::vector<sc_hit> hits;
+};
+
+namespace {
+
+sc_status dev_alloc(sc_dd* dd, void** p, size_t bytes, const char* what) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(e == hipErrorOutOfMemory ? SC_ERR_OUT_OF_MEMORY : SC_ERR_HIP,
+                std::string("hipMalloc ") + what + ": " + hipGetErrorString(e));
+  }
+  dd->device_bytes += bytes;
+  return SC_OK;
+}
+
+sc_status host_alloc(void** p, size_t bytes, const char* what) {
+  hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(SC_ERR_OUT_OF_MEMORY, std::string("hipHostMalloc ") + what + ": " +
+                                          hipGetErrorString(e));
+  }
+  return SC_OK;
+}
+
+void release(sc_dd* dd) {
+  hipFree(dd->top_key);
+  hipFree(dd->colsum);
+  hipFree(dd->top_sum);
+  hipFree(dd->top_blk);
+  hipFree(dd->top_off);
+  hipFree(dd->stats_scratch);
+  hipFree(dd->stats_out);
+  hipFree(dd->cand);
+  hipFree(dd->tree_a);
+  hipFree(dd->tree_b);
+  hipFree(dd->gather_idx);
+  hipFree(dd->gather_out);
+  hipHostFree(dd->h_colsum);
+  hipHostFree(dd->h_top_sum);
+  hipHostFree(dd->h_top_blk);
+  hipHostFree(dd->h_top_off);
+  hipHostFree(dd->h_cand);
+  hipHostFree(dd->h_stats);
+  if (dd->ev_start) hipEventDestroy(dd->ev_start);
+  if (dd->ev_stop) hipEventDestroy(dd->ev_stop);
+}
+
+struct DriftRange {
+  int bmin, bmax, window;
+  double nmd;
+};
+
+// dedoppler.cu:195-198, 274.
+DriftRange drift_range(const sc_dd* dd, double max_drift) {
+  DriftRange r;
+  double diagonal = dd->resolution * dd->drift_timesteps;
+  r.nmd = max_drift / std::fabs(diagonal);
+  r.bmin = (int)std::floor(-r.nmd);
+  r.bmax = (int)std::floor(r.nmd);
+  r.window = (int)(2 * std::ceil(r.nmd * dd->drift_timesteps));
+  return r;
+}
+
+sc_status ensure_tree_scratch(sc_dd* dd) {
+  if (dd->tree_a) return SC_OK;
+  size_t bytes = sizeof(float) * (size_t)dd->rounded * dd->num_channels;
+  sc_status s = dev_alloc(dd, (void**)&dd->tree_a, bytes, "tree scratch");
+  if (s != SC_OK) return s;
+  return dev_alloc(dd, (void**)&dd->tree_b, bytes, "tree scratch");
+}
+
+// Path sums of one drift block into a [T_r][N] buffer; returns which buffer.
+sc_status run_tree(sc_dd* dd, const void* data, int dtype, long stride, int drift,
+                   float** result, hipStream_t stream) {
+  sc_status s = ensure_tree_scratch(dd);
+  if (s != SC_OK) return s;
+  const int T = dd->rounded, N = dd->num_channels;
+  SC_HIP(sc::launch_tree_blocks(data, dtype, stride, T, N, drift, dd->tree_a, stream),
+         "tree blocks");
+  float* src = dd->tree_a;
+  float* dst = dd->tree_b;
+  for (int L = 32; L <= T; L *= 2) {
+    SC_HIP(sc::launch_one_step(src, dst, T, N, L, drift, stream), "one step");
+    std::swap(src, dst);
+  }
+  *result = src;
+  return SC_OK;
+}
+
+// dedoppler.cu:241-262, same calls on a copy of the column sums.
+void host_exact_stats(const float* colsums, int n, float* median_out, float* std_out) {
+  std::vector<float> v(colsums, colsums + n);
+  float* sums = v.data();
+  int mid = n / 2;
+  float* end = sums + n;
+  std::nth_element(sums, sums + mid, end);
+  int first = (int)std::ceil(0.05 * n);
+  int last = (int)std::floor(0.95 * n);
+  std::nth_element(sums, sums + first, sums + mid - 1);
+  std::nth_element(sums + mid + 1, sums + last, end);
+  float median = sums[mid];
+  float sum = std::accumulate(sums + first, sums + last + 1, 0.0);
+  float m = sum / (last + 1 - first);
+  float accum = 0.0;
+  std::for_each(sums + first, sums + last + 1,
+                [&](const float f) { accum += (f - m) * (f - m); });
+  float std_dev = std::sqrt(accum / (last + 1 - first));
+  *median_out = median;
+  *std_out = std_dev;
+}
+
+// DedopplerHit constructor (dedoppler_hit.cpp:11-23).
+sc_hit make_hit(const sc_meta* meta, int index, int drift_steps, double drift_rate, float snr,
+                int beam, int coarse_channel, int num_timesteps, float power) {
+  sc_hit h;
+  h.index = index;
+  h.drift_steps = drift_steps;
+  h.drift_rate = drift_rate;
+  h.snr = snr;
+  h.coarse_channel = coarse_channel;
+  h.beam = (beam >= 0 && beam < meta->num_coherent_beams) ? beam : -1;
+  h.num_timesteps = num_timesteps;
+  h.power = power;
+  h.incoherent_power = 0.0f;
+  int coarse_offset = (int)(coarse_channel * meta->coarse_channel_size);
+  int global_index = coarse_offset + index;
+  h.frequency = meta->fch1 + global_index * meta->foff;
+  return h;
+}
+
+// dedoppler.cu:264-319 on host arrays.
+void host_merge(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
+                int coarse_channel, double min_drift, double snr_threshold, float median,
+                float std_dev) {
+  const int N = dd->num_channels;
+  const float* top = dd->h_top_sum;
+  float threshold = snr_threshold * std_dev + median;
+  const int window = r.window;
+  for (int i = 0; (long)i * window < N; ++i) {
+    int cand = -1;
+    float cand_sum = threshold;
+    for (int j = 0; j < window; ++j) {
+      int f = i * window + j;
+      if (f >= N) break;
+      if (top[f] > cand_sum) {
+        cand = f;
+        cand_sum = top[f];
+      }
+    }
+    if (cand < 0) continue;
+    int wend = std::min(N, cand + window);
+    bool larger = false;
+    for (int f = std::max(0, cand - window + 1); f < wend; ++f) {
+      if (top[f] > cand_sum) {
+        larger = true;
+        break;
+      }
+    }
+    if (larger) continue;
+    int drift_bins = dd->h_top_blk[cand] * dd->drift_timesteps + dd->h_top_off[cand];
+    double drift_rate = drift_bins * dd->resolution;
+    float snr = (cand_sum - median) / std_dev;
+    if (std::fabs(drift_rate) >= min_drift) {
+      dd->hits.push_back(make_hit(meta, cand, drift_bins, drift_rate, snr, beam, coarse_channel,
+                                  dd->num_timesteps, cand_sum));
+    }
+  }
+}
+
+sc_status validate_input(const sc_dd* dd, const void* data, int dtype, int rounded, int N,
+                         long stride) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  if (!data) return fail(SC_ERR_INVALID_ARGUMENT, "null data");
+  if (dtype != SC_F32 && dtype != SC_F16 && dtype != SC_BF16)
+    return fail(SC_ERR_INVALID_ARGUMENT, "unknown dtype");
+  // dedoppler.cu:191-192
+  if (rounded != dd->rounded)
+    return fail(SC_ERR_INVALID_ARGUMENT, "input.num_timesteps != rounded_num_timesteps (" +
+                                             std::to_string(rounded) + " vs " +
+                                             std::to_string(dd->rounded) + ")");
+  if (N != dd->num_channels)
+    return fail(SC_ERR_INVALID_ARGUMENT, "input.num_channels != num_channels");
+  if (stride < N) return fail(SC_ERR_INVALID_ARGUMENT, "row_stride < num_channels");
+  return SC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sc_last_error_string(void) { return sc::last_error(); }
+
+int sc_abi_version(void) { return SC_ABI_VERSION; }
+
+sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, double tsamp_s,
+                       int has_dc_spike, sc_dd** out) {
+  if (!out) return fail(SC_ERR_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  // dedoppler.cu:83
+  if (num_timesteps <= 1) return fail(SC_ERR_INVALID_ARGUMENT, "num_timesteps must be > 1");
+  if (num_channels <= 0) return fail(SC_ERR_INVALID_ARGUMENT, "num_channels must be > 0");
+  if (!(tsamp_s > 0) || foff_mhz == 0 || !std::isfinite(foff_mhz))
+    return fail(SC_ERR_INVALID_ARGUMENT, "foff must be nonzero and tsamp positive");
+  int rounded = round_up_pow2(num_timesteps);
+  if (rounded > 4096) return fail(SC_ERR_UNSUPPORTED, "more than 4096 time steps");
+  if ((long)rounded * num_channels > (1L << 31))
+    return fail(SC_ERR_UNSUPPORTED, "T_r * N above 2^31 samples");
+
+  sc_dd* dd = new sc_dd();
+  dd->num_timesteps = num_timesteps;
+  dd->rounded = rounded;
+  dd->num_channels = num_channels;
+  dd->foff = foff_mhz;
+  dd->tsamp = tsamp_s;
+  dd->has_dc_spike = has_dc_spike != 0;
+  dd->drift_timesteps = rounded - 1;
+  // dedoppler.cu:87
+  dd->resolution = 1e6 * foff_mhz / (dd->drift_timesteps * tsamp_s);
+  default_tuning(rounded, &dd->lanes, &dd->group);
+  hipGetDevice(&dd->device);
+
+  const size_t N = num_channels;
+  sc_status s = SC_OK;
+  if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->top_key, N * sizeof(sc::top_key_t), "top key");
+  if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->colsum, N * sizeof(float), "column sums");
+  if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->top_sum, N * sizeof(float), "top sums");
+  if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->top_blk, N * sizeof(int), "top blocks");
+  if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->top_off, N * sizeof(int), "top offsets");
+  if (s == SC_OK)
+    s = dev_alloc(dd, &dd->stats_scratch, sc::device_stats_scratch_bytes(num_channels),
+                  "stats scratch");
+  if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->stats_out, 4 * sizeof(float), "stats");
+  // At most one candidate per window and window >= 2 (max_drift > 0).
+  dd->cand_capacity = (N + 1) / 2 + 1;
+  if (s == SC_OK)
+    s = dev_alloc(dd, (void**)&dd->cand, dd->cand_capacity * sizeof(int4), "candidates");
+  if (s == SC_OK) s = host_alloc((void**)&dd->h_colsum, N * sizeof(float), "column sums");
+  if (s == SC_OK) s = host_alloc((void**)&dd->h_top_sum, N * sizeof(float), "top sums");
+  if (s == SC_OK) s = host_alloc((void**)&dd->h_top_blk, N * sizeof(int), "top blocks");
+  if (s == SC_OK) s = host_alloc((void**)&dd->h_top_off, N * sizeof(int), "top offsets");
+  if (s == SC_OK)
+    s = host_alloc((void**)&dd->h_cand, dd->cand_capacity * sizeof(int4), "candidates");
+  if (s == SC_OK) s = host_alloc((void**)&dd->h_stats, 4 * sizeof(float), "stats");
+  if (s == SC_OK && rounded > 16) s = ensure_tree_scratch(dd);
+  if (s == SC_OK) {
+    // Block/offset arrays persist across searches like the reference's
+    // (dedoppler.cu:205 only resets the sums); start them at zero.
+    hipError_t e = hipMemset(dd->top_blk, 0, N * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(dd->top_off, 0, N * sizeof(int));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) s = hip_fail(e, "initialise top arrays");
+  }
+  if (s != SC_OK) {
+    release(dd);
+    delete dd;
+    return s;
+  }
+  *out = dd;
+  return SC_OK;
+}
+
+sc_status sc_dd_destroy(sc_dd* dd) {
+  if (!dd) return SC_OK;
+  release(dd);
+  delete dd;
+  return SC_OK;
+}
+
+sc_status sc_dd_memory_usage(const sc_dd* dd, size_t* bytes) {
+  if (!dd || !bytes) return fail(SC_ERR_INVALID_ARGUMENT, "null argument");
+  *bytes = dd->device_bytes;
+  return SC_OK;
+}
+
+sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  if (mode != SC_STATS_HOST_EXACT && mode != SC_STATS_DEVICE)
+    return fail(SC_ERR_INVALID_ARGUMENT, "unknown stats mode");
+  dd->stats_mode = mode;
+  return SC_OK;
+}
+
+sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  if (dd->rounded > 16) return fail(SC_ERR_UNSUPPORTED, "no fused kernel above 16 time steps");
+  if (!sc::fused_supported(dd->rounded, channels_per_lane, drift_group))
+    return fail(SC_ERR_UNSUPPORTED, "no fused kernel for this (lanes, group) shape");
+  dd->lanes = channels_per_lane;
+  dd->group = drift_group;
+  return SC_OK;
+}
+
+sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_timesteps,
+                       int num_channels, long row_stride, const sc_meta* meta, int beam,
+                       int coarse_channel, double max_drift, double min_drift,
+                       double snr_threshold, sc_hit* hits, size_t capacity, size_t* num_hits,
+                       void* stream_ptr) {
+  sc_status s = validate_input(dd, data, dtype, rounded_timesteps, num_channels, row_stride);
+  if (s != SC_OK) return s;
+  if (!meta) return fail(SC_ERR_INVALID_ARGUMENT, "null metadata");
+  if (!(max_drift > 0) || !std::isfinite(max_drift))
+    return fail(SC_ERR_INVALID_ARGUMENT,
+                "max_drift must be > 0 (the reference loops forever at 0, dedoppler.cu:275)");
+  hipStream_t stream = (hipStream_t)stream_ptr;
+  const int T = dd->rounded, N = dd->num_channels;
+  const DriftRange r = drift_range(dd, max_drift);
+  if (r.window <= 0) return fail(SC_ERR_INVALID_ARGUMENT, "window size is 0");
+  const int nblocks = r.bmax - r.bmin + 1;
+  // Drift codes must fit the 29 key bits (dd_internal.h).
+  if ((long)nblocks * T > 0x1FFFFFFFL) return fail(SC_ERR_UNSUPPORTED, "drift range too large");
+  dd->hits.clear();
+  dd->state_valid = false;
+
+  // dedoppler.cu:205 (the reference zeroes the float sums; the key 0 means "no
+  // positive path").
+  SC_HIP(sc::launch_fill_u64(dd->top_key, N, 0ull, stream), "reset top keys");
+  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_start, stream), "timing event");
+  if (T <= 16 && sc::fused_supported(T, dd->lanes, dd->group)) {
+    SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
+                                     dd->group,
+                                     dd->top_key, dd->colsum, stream),
+           "fused tree");
+  } else {
+    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
+    for (int b = r.bmin; b <= r.bmax; ++b) {
+      float* sums = nullptr;
+      s = run_tree(dd, data, dtype, row_stride, b, &sums, stream);
+      if (s != SC_OK) return s;
+      SC_HIP(sc::launch_top_update(sums, T, N, b, r.bmin, dd->top_key, stream), "top update");
+    }
+  }
+  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
+  SC_HIP(sc::launch_unpack_top(dd->top_key, N, T, r.bmin, dd->top_sum, dd->top_blk, dd->top_off,
+                               stream),
+         "unpack top");
+
+  if (dd->stats_mode == SC_STATS_HOST_EXACT) {
+    // dedoppler.cu:232-240
+    SC_HIP(hipMemcpyAsync(dd->h_colsum, dd->colsum, N * sizeof(float), hipMemcpyDeviceToHost,
+                          stream), "copy column sums");
+    SC_HIP(hipMemcpyAsync(dd->h_top_sum, dd->top_sum, N * sizeof(float), hipMemcpyDeviceToHost,
+                          stream), "copy top sums");
+    SC_HIP(hipMemcpyAsync(dd->h_top_blk, dd->top_blk, N * sizeof(int), hipMemcpyDeviceToHost,
+                          stream), "copy top blocks");
+    SC_HIP(hipMemcpyAsync(dd->h_top_off, dd->top_off, N * sizeof(int), hipMemcpyDeviceToHost,
+                          stream), "copy top offsets");
+    SC_HIP(hipStreamSynchronize(stream), "search");
+    float median, std_dev;
+    host_exact_stats(dd->h_colsum, N, &median, &std_dev);
+    dd->last_median = median;
+    dd->last_std = std_dev;
+    host_merge(dd, r, meta, beam, coarse_channel, min_drift, snr_threshold, median, std_dev);
+  } else {
+    SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_out, stream),
+           "device stats");
+    SC_HIP(sc::launch_window_merge(dd->top_sum, dd->top_blk, dd->top_off, N, r.window,
+                                   snr_threshold, dd->stats_out, dd->cand, stream),
+           "window merge");
+    const int nwin = (int)(((long)N + r.window - 1) / r.window);
+    SC_HIP(hipMemcpyAsync(dd->h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost,
+                          stream), "copy candidates");
+    SC_HIP(hipMemcpyAsync(dd->h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
+                          stream), "copy stats");
+    SC_HIP(hipStreamSynchronize(stream), "search");
+    const float median = dd->h_stats[0], std_dev = dd->h_stats[1];
+    dd->last_median = median;
+    dd->last_std = std_dev;
+    for (int i = 0; i < nwin; ++i) {
+      const int4 c = dd->h_cand[i];
+      if (c.x < 0) continue;
+      float cand_sum;
+      std::memcpy(&cand_sum, &c.y, sizeof(float));
+      int drift_bins = c.z * dd->drift_timesteps + c.w;
+      double drift_rate = drift_bins * dd->resolution;
+      float snr = (cand_sum - median) / std_dev;
+      if (std::fabs(drift_rate) >= min_drift)
+        dd->hits.push_back(make_hit(meta, c.x, drift_bins, drift_rate, snr, beam, coarse_channel,
+                                    dd->num_timesteps, cand_sum));
+    }
+  }
+  if (dd->timing) {
+    float ms = 0.0f;
+    SC_HIP(hipEventElapsedTime(&ms, dd->ev_start, dd->ev_stop), "timing");
+    dd->kernel_ms += ms;
+    dd->kernel_launches += 1;
+  }
+  dd->state_valid = true;
+  if (num_hits) *num_hits = dd->hits.size();
+  if (hits && capacity) {
+    size_t n = std::min(capacity, dd->hits.size());
+    std::memcpy(hits, dd->hits.data(), n * sizeof(sc_hit));
+  }
+  return SC_OK;
+}
+
+sc_status sc_dd_set_timing(sc_dd* dd, int enable) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  if (enable && !dd->ev_start) {
+    SC_HIP(hipEventCreate(&dd->ev_start), "event");
+    SC_HIP(hipEventCreate(&dd->ev_stop), "event");
+  }
+  dd->timing = enable != 0;
+  dd->kernel_ms = 0.0;
+  dd->kernel_launches = 0;
+  return SC_OK;
+}
+
+sc_status sc_dd_kernel_time(const sc_dd* dd, double* total_ms, long* searches) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  if (total_ms) *total_ms = dd->kernel_ms;
+  if (searches) *searches = dd->kernel_launches;
+  return SC_OK;
+}
+
+sc_status sc_dd_copy_hits(const sc_dd* dd, sc_hit* hits, size_t capacity, size_t* num_hits) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  if (num_hits) *num_hits = dd->hits.size();
+  if (hits && capacity) {
+    size_t n = std::min(capacity, dd->hits.size());
+    std::memcpy(hits, dd->hits.data(), n * sizeof(sc_hit));
+  }
+  return SC_OK;
+}
+
+sc_status sc_dd_copy_state(const sc_dd* dd, float* column_sums, float* top_sums,
+                           int* top_drift_blocks, int* top_path_offsets, float* median,
+                           float* std_dev) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  if (!dd->state_valid) return fail(SC_ERR_INVALID_ARGUMENT, "no completed search");
+  const size_t N = dd->num_channels;
+  if (column_sums) SC_HIP(hipMemcpy(column_sums, dd->colsum, N * 4, hipMemcpyDeviceToHost), "copy");
+  if (top_sums) SC_HIP(hipMemcpy(top_sums, dd->top_sum, N * 4, hipMemcpyDeviceToHost), "copy");
+  if (top_drift_blocks)
+    SC_HIP(hipMemcpy(top_drift_blocks, dd->top_blk, N * 4, hipMemcpyDeviceToHost), "copy");
+  if (top_path_offsets)
+    SC_HIP(hipMemcpy(top_path_offsets, dd->top_off, N * 4, hipMemcpyDeviceToHost), "copy");
+  if (median) *median = dd->last_median;
+  if (std_dev) *std_dev = dd->last_std;
+  return SC_OK;
+}
+
+sc_status sc_dd_taylor_tree(sc_dd* dd, const void* data, int dtype, int rounded_timesteps,
+                            int num_channels, long row_stride, int drift_block, float* out,
+                            void* stream_ptr) {
+  sc_status s = validate_input(dd, data, dtype, rounded_timesteps, num_channels, row_stride);
+  if (s != SC_OK) return s;
+  if (!out) return fail(SC_ERR_INVALID_ARGUMENT, "null output");
+  hipStream_t stream = (hipStream_t)stream_ptr;
+  float* sums = nullptr;
+  s = run_tree(dd, data, dtype, row_stride, drift_block, &sums, stream);
+  if (s != SC_OK) return s;
+  SC_HIP(hipMemcpyAsync(out, sums, sizeof(float) * (size_t)dd->rounded * dd->num_channels,
+                        hipMemcpyDeviceToDevice, stream),
+         "copy tree");
+  SC_HIP(hipStreamSynchronize(stream), "tree");
+  return SC_OK;
+}
+
+sc_status sc_dd_add_incoherent_power(sc_dd* dd, const void* data, int dtype,
+                                     int rounded_timesteps, int num_channels, long row_stride,
+                                     sc_hit* hits, size_t num_hits, void* stream_ptr) {
+  sc_status s = validate_input(dd, data, dtype, rounded_timesteps, num_channels, row_stride);
+  if (s != SC_OK) return s;
+  if (num_hits && !hits) return fail(SC_ERR_INVALID_ARGUMENT, "null hits");
+  hipStream_t stream = (hipStream_t)stream_ptr;
+  // dedoppler.cu:144: sort by drift steps (same comparator, same std::sort).
+  std::sort(hits, hits + num_hits, [](const sc_hit& a, const sc_hit& b) {
+    return a.drift_steps < b.drift_steps;
+  });
+  if (num_hits > dd->gather_capacity) {
+    hipFree(dd->gather_idx);
+    hipFree(dd->gather_out);
+    dd->gather_idx = nullptr;
+    dd->gather_out = nullptr;
+    s = dev_alloc(dd, (void**)&dd->gather_idx, 2 * num_hits * sizeof(int), "gather");
+    if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->gather_out, num_hits * sizeof(float), "gather");
+    if (s != SC_OK) return s;
+    dd->gather_capacity = num_hits;
+  }
+  const int drift_shift = dd->rounded - 1;
+  size_t i = 0;
+  while (i < num_hits) {
+    // dedoppler.cu:156-157: block by float floor division.
+    int block = (int)std::floor((float)hits[i].drift_steps / drift_shift);
+    size_t j = i;
+    std::vector<int> idx;
+    while (j < num_hits &&
+           (int)std::floor((float)hits[j].drift_steps / drift_shift) == block) {
+      int offset = hits[j].drift_steps - block * drift_shift;
+      if (offset < 0 || offset >= drift_shift || hits[j].index < 0 ||
+          hits[j].index >= dd->num_channels)
+        return fail(SC_ERR_INVALID_ARGUMENT, "hit outside the searched range");
+      idx.push_back(offset);
+      ++j;
+    }
+    const int n = (int)(j - i);
+    for (int k = 0; k < n; ++k) idx.push_back(hits[i + k].index);
+    float* sums = nullptr;
+    s = run_tree(dd, data, dtype, row_stride, block, &sums, stream);
+    if (s != SC_OK) return s;
+    SC_HIP(hipMemcpyAsync(dd->gather_idx, idx.data(), 2 * n * sizeof(int),
+                          hipMemcpyHostToDevice, stream), "copy indices");
+    SC_HIP(sc::launch_gather(sums, dd->num_channels, dd->gather_idx, dd->gather_idx + n, n,
+                             dd->gather_out, stream), "gather");
+    std::vector<float> vals(n);
+    SC_HIP(hipMemcpyAsync(vals.data(), dd->gather_out, n * sizeof(float), hipMemcpyDeviceToHost,
+                          stream), "copy values");
+    SC_HIP(hipStreamSynchronize(stream), "incoherent power");
+    for (int k = 0; k < n; ++k) hits[i + k].incoherent_power = vals[k];
+    i = j;
+  }
+  return SC_OK;
+}
+
+sc_status sc_normalize(const float* data, int rounded_timesteps, int num_channels,
+                       long row_stride, int out_dtype, float clamp, void* out, double* mean_out,
+                       double* std_out, void* stream_ptr) {
+  if (!data || !out) return fail(SC_ERR_INVALID_ARGUMENT, "null pointer");
+  if (rounded_timesteps <= 0 || num_channels <= 0 || row_stride < num_channels)
+    return fail(SC_ERR_INVALID_ARGUMENT, "bad shape");
+  if (out_dtype != SC_F32 && out_dtype != SC_F16 && out_dtype != SC_BF16)
+    return fail(SC_ERR_INVALID_ARGUMENT, "unknown dtype");
+  hipStream_t stream = (hipStream_t)stream_ptr;
+  void* scratch = nullptr;
+  double* moments = nullptr;
+  SC_HIP(hipMalloc(&scratch, sc::normalize_scratch_bytes(rounded_timesteps, num_channels)),
+         "normalize scratch");
+  hipError_t e = hipMalloc((void**)&moments, 2 * sizeof(double));
+  if (e != hipSuccess) {
+    hipFree(scratch);
+    return hip_fail(e, "normalize moments");
+  }
+  e = sc::launch_normalize(data, rounded_timesteps, num_channels, row_stride, out_dtype, clamp,
+                           out, moments, scratch, stream);
+  double h[2] = {0, 0};
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h, moments, sizeof(h), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  hipFree(scratch);
+  hipFree(moments);
+  if (e != hipSuccess) return hip_fail(e, "normalize");
+  if (mean_out) *mean_out = h[0];
+  if (std_out) *std_out = h[1];
+  return SC_OK;
+}
+
+sc_status sc_gen_waterfall(float* out, int num_timesteps, int rounded_timesteps,
+                           int num_channels, uint64_t seed, float gain, float taper_frac,
+                           float dc_spike, int num_tones, const float* tone_start,
+                           const float* tone_slope, const float* tone_amp, void* stream_ptr) {
+  if (!out) return fail(SC_ERR_INVALID_ARGUMENT, "null output");
+  if (num_timesteps <= 0 || rounded_timesteps < num_timesteps || num_channels <= 0)
+    return fail(SC_ERR_INVALID_ARGUMENT, "bad shape");
+  if (num_tones < 0 || (num_tones > 0 && (!tone_start || !tone_slope || !tone_amp)))
+    return fail(SC_ERR_INVALID_ARGUMENT, "bad tone list");
+  hipStream_t stream = (hipStream_t)stream_ptr;
+  float* dev = nullptr;
+  if (num_tones > 0) {
+    SC_HIP(hipMalloc((void**)&dev, 3 * sizeof(float) * num_tones), "tone params");
+    std::vector<float> h(3 * (size_t)num_tones);
+    std::memcpy(h.data(), tone_start, sizeof(float) * num_tones);
+    std::memcpy(h.data() + num_tones, tone_slope, sizeof(float) * num_tones);
+    std::memcpy(h.data() + 2 * num_tones, tone_amp, sizeof(float) * num_tones);
+    hipError_t e = hipMemcpy(dev, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(dev);
+      return hip_fail(e, "tone params");
+    }
+  }
+  hipError_t e = sc::launch_gen_waterfall(out, num_timesteps, rounded_timesteps, num_channels,
+                                          seed, gain, taper_frac, dc_spike, num_tones, dev,
+                                          dev ? dev + num_tones : nullptr,
+                                          dev ? dev + 2 * num_tones : nullptr, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  hipFree(dev);
+  if (e != hipSuccess) return hip_fail(e, "generate waterfall");
+  return SC_OK;
+}
+
+}  // extern "C"

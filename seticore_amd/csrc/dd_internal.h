@@ -1,0 +1,85 @@
+// Internal interface between the C-ABI host code (dd_capi.cpp) and the HIP
+// kernels (dd_kernels.hip). Not installed; not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sc {
+
+// Per-channel search state: the bits of an FP64 key whose value orders
+// (top path sum, -code) lexicographically - the float sum widened to double
+// with 0x1FFFFFFF - code in the 29 low mantissa bits, where
+// code = (drift_block - min_drift_block) * T_r + path_offset. A max keeps the
+// largest sum and, on equal sums, the lowest (drift block, offset): the
+// reference's strict ">" in ascending order (dedoppler.cu:40,49,214).
+// 0 means "no path sum > 0" (dedoppler.cu:205).
+typedef unsigned long long top_key_t;
+
+// Fused tree + top-path tracking for T_r <= 16 (all drift blocks, one interior
+// launch plus one launch over the edge tiles). C = channels per lane,
+// G = drift blocks per LDS window. Writes column sums for every channel and
+// max-merges top keys.
+hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, int N,
+                                 int bmin, int nblocks, int C, int G, top_key_t* top_key,
+                                 float* colsum, hipStream_t stream);
+
+// Whether launch_fused_tree_top has a kernel for (T, C, G).
+bool fused_supported(int T, int C, int G);
+
+// Level-min(T,16) sums for every time block of one drift block:
+// out[(tb*Tb + p)*N + c], Tb = min(T, 16), tb < T/Tb.
+hipError_t launch_tree_blocks(const void* x, int dtype, long stride, int T, int N,
+                              int drift, float* out, hipStream_t stream);
+
+// One global-memory recursion stage (taylor.cu:97-109 semantics).
+hipError_t launch_one_step(const float* src, float* dst, int T, int N, int L, int drift,
+                           hipStream_t stream);
+
+// Sequential per-drift-block top update from a [T][N] path-sum buffer.
+hipError_t launch_top_update(const float* sums, int T, int N, int drift, int bmin,
+                             top_key_t* top_key, hipStream_t stream);
+
+// Column sums (dedoppler.cu:63-72 order) for the paths that do not fuse them.
+hipError_t launch_colsum(const void* x, int dtype, long stride, int T, int N, float* colsum,
+                         hipStream_t stream);
+
+hipError_t launch_fill_u64(top_key_t* p, long n, top_key_t value, hipStream_t stream);
+
+// Unpacks keys into the reference's three top arrays; channels without a
+// positive path get top_sum 0 and keep their previous block/offset.
+hipError_t launch_unpack_top(const top_key_t* key, int N, int T, int bmin, float* top_sum,
+                             int* top_blk, int* top_off, hipStream_t stream);
+
+// ---- device statistics -----------------------------------------------------
+// Scratch layout (device): see DeviceStatsScratch in dd_kernels.hip.
+size_t device_stats_scratch_bytes(int N);
+// Computes median (rank N/2), and the trimmed mean/std over sorted ranks
+// [ceil(.05N), floor(.95N)] into stats_out[0..1] = {median, std} (float), plus
+// stats_d[0..2] = {mean, std, median} in double for diagnostics.
+hipError_t launch_device_stats(const float* colsum, int N, void* scratch, float* stats_out,
+                               hipStream_t stream);
+
+// Per-window merge (dedoppler.cu:273-303). cand_out[i] = {index or -1, sum bits,
+// drift block, path offset} for window i.
+hipError_t launch_window_merge(const float* top_sum, const int* top_blk, const int* top_off,
+                               int N, int window, double snr_threshold,
+                               const float* stats /* {median, std} */, int4* cand_out,
+                               hipStream_t stream);
+
+// Gathers path sums for (offset, channel) pairs from a [T][N] buffer.
+hipError_t launch_gather(const float* sums, int N, const int* offsets, const int* chans,
+                         int n, float* out, hipStream_t stream);
+
+// ---- normalisation and synthetic data -------------------------------------
+hipError_t launch_normalize(const float* x, int T, int N, long stride, int out_dtype,
+                            float clamp, void* out, double* moments /* device, 2 */,
+                            void* scratch, hipStream_t stream);
+size_t normalize_scratch_bytes(int T, int N);
+
+hipError_t launch_gen_waterfall(float* out, int T, int T_r, int N, uint64_t seed, float gain,
+                                float taper_frac, float dc_spike, int num_tones,
+                                const float* tone_start, const float* tone_slope,
+                                const float* tone_amp, hipStream_t stream);
+
+}  // namespace sc

@@ -1,0 +1,263 @@
+"""GPU parity: the HIP search through the C ABI against the CPU oracle.
+
+Bar: bit-exact path sums, top sums, drift blocks/offsets, column sums and hits
+(host-exact statistics). The device-statistics mode must give the exact median,
+a std within 1e-5 relative of the oracle's, and the same hit set except for
+candidates whose top sum lies within the threshold difference (documented
+ties at the SNR threshold).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import seticore_amd as sc
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _to_gpu(arr):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(arr)).to("cuda")
+
+
+def _waterfall(T, N, seed, tones=(), gain=1.0, T_valid=None):
+    """Seeded exponential noise + tones (numpy, host side)."""
+    rng = np.random.default_rng(seed)
+    T_valid = T if T_valid is None else T_valid
+    data = np.zeros((T, N), np.float32)
+    data[:T_valid] = rng.exponential(gain, size=(T_valid, N)).astype(np.float32)
+    for start, slope, amp in tones:
+        for t in range(T_valid):
+            c = int(math.floor(start + slope * t + 0.5))
+            if 0 <= c < N:
+                data[t, c] += np.float32(amp)
+    return data
+
+
+def _hits_as_tuples(hits):
+    return [(h["index"] if isinstance(h, dict) else h.index,
+             h["drift_steps"] if isinstance(h, dict) else h.drift_steps)
+            for h in hits]
+
+
+def _hit_fields(h):
+    names = ["frequency", "index", "drift_steps", "drift_rate", "snr", "coarse_channel", "beam",
+             "num_timesteps", "power", "incoherent_power"]
+    if isinstance(h, dict):
+        return tuple(h[n] for n in names)
+    return tuple(getattr(h, n) for n in names)
+
+
+# --- taylor_test.cu:8-51: tree outputs over valid paths ---------------------
+@pytest.mark.parametrize("T", [2, 4, 8, 16, 32, 64, 128, 256])
+def test_tree_matches_oracle(gpu, oracle, T):
+    N = 1000
+    print("tree T=%d N=%d" % (T, N), flush=True)
+    t = np.arange(T)[:, None]
+    c = np.arange(N)[None, :]
+    ramp = (1.0 * (1 + t + c)).astype(np.float32)
+    noise = _waterfall(T, N, seed=T)
+    dd = sc.Dedopplerer(T, N, 1.0, 1.0)
+    for data in (ramp, noise):
+        buf = sc.FilterbankBuffer(T, N, data=_to_gpu(data))
+        for b in range(-2, 3):
+            got = dd.taylor_tree(buf, b).cpu().numpy()
+            want = oracle.basic_taylor_tree(data, b)
+            p = np.arange(T)[:, None]
+            cc = np.arange(N)[None, :]
+            last = cc + (T - 1) * b + p
+            valid = (last >= 0) & (last < N)
+            np.testing.assert_array_equal(got[valid], want[valid])
+
+
+# --- dedoppler_test.cpp:9-33 ------------------------------------------------
+def test_known_answer(gpu):
+    T, N = 8, 1000
+    buf = sc.makeNoisyBuffer(T, N)
+    for t, ch in enumerate([70, 70, 71, 71, 72, 72, 73, 73]):
+        buf.set(t, ch, 1.0)
+    meta = sc.FilterbankMetadata()
+    dd = sc.Dedopplerer(T, N, 1.0, 1.0, False)
+    hits = []
+    dd.search(buf, meta, sc.NO_BEAM, 555, 0.01, 0.01, 200.0, hits)
+    assert len(hits) == 1
+    assert hits[0].index == 70
+    assert hits[0].drift_steps == 3
+    assert hits[0].beam == sc.NO_BEAM
+    assert hits[0].coarse_channel == 555
+
+
+def _compare_search(oracle, data, T_unrounded, foff, tsamp, max_drift, min_drift, snr,
+                     stats_mode=sc.SC_STATS_HOST_EXACT, tuning=None, dtype=sc.SC_F32):
+    T, N = data.shape
+    dd = sc.Dedopplerer(T_unrounded, N, foff, tsamp)
+    dd.set_stats_mode(stats_mode)
+    if tuning:
+        dd.set_tuning(*tuning)
+    buf = sc.FilterbankBuffer(T, N, data=_to_gpu(data))
+    meta = sc.FilterbankMetadata(fch1=8000.0, foff=foff, coarse_channel_size=N,
+                                 source_names=["a", "b"])
+    hits = []
+    dd.search(buf, meta, 1, 3, max_drift, min_drift, snr, hits)
+    st = dd.state()
+    ref = oracle.search(data, T_unrounded, foff, tsamp, max_drift, min_drift, snr, fch1=8000.0,
+                        num_coherent_beams=2, beam=1, coarse_channel=3, num_threads=8)
+    return hits, st, ref
+
+
+CASES = [
+    # (T_unrounded, N, foff, tsamp, max_drift, snr, seed)
+    (16, 4099, -2.79e-6, 18.25, 1.0, 6.0, 1),
+    (16, 65536, -2.79e-6, 18.25, 4.0, 6.0, 2),
+    (13, 10000, 2.79e-6, 18.25, 2.0, 5.0, 3),   # padded rows, positive foff
+    (8, 30000, -2.79e-6, 9.0, 3.0, 6.0, 4),
+    (4, 20000, 1.0, 1.0, 1e6, 5.0, 5),
+    (2, 5000, 1.0, 1.0, 2e6, 4.0, 6),
+    (32, 20000, -2.79e-6, 18.25, 1.0, 6.0, 7),
+    (64, 8192, -2.79e-6, 18.25, 0.5, 6.0, 8),
+    (16, 1000, -2.79e-6, 18.25, 10.0, 4.0, 9),  # drift range wider than the channel
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(c[:2]) for c in CASES])
+def test_search_parity_host_exact(gpu, oracle, case):
+    Tu, N, foff, tsamp, max_drift, snr, seed = case
+    T = sc.round_up_to_power_of_two(Tu)
+    print("search case", case, flush=True)
+    rng = np.random.default_rng(seed)
+    tones = [(rng.uniform(0, N), rng.uniform(-3, 3), rng.uniform(1, 4)) for _ in range(8)]
+    data = _waterfall(T, N, seed, tones, T_valid=Tu)
+    hits, st, ref = _compare_search(oracle, data, Tu, foff, tsamp, max_drift, 0.0, snr)
+    np.testing.assert_array_equal(st["column_sums"], ref.column_sums)
+    np.testing.assert_array_equal(st["top_sums"], ref.top_sums)
+    pos = ref.top_sums > 0
+    np.testing.assert_array_equal(st["top_blocks"][pos], ref.top_blocks[pos])
+    np.testing.assert_array_equal(st["top_offsets"][pos], ref.top_offsets[pos])
+    assert st["median"] == ref.median and st["std"] == ref.std
+    assert len(hits) == len(ref.hits)
+    for a, b in zip(hits, ref.hits):
+        assert _hit_fields(a) == _hit_fields(b)
+
+
+@pytest.mark.parametrize("tuning", [(4, 2), (4, 4), (4, 8), (8, 2), (8, 4)])
+def test_tunings_are_bit_identical(gpu, oracle, tuning):
+    T, N = 16, 50000
+    data = _waterfall(T, N, 21, [(1000.5, 1.3, 3.0), (30000.0, -2.2, 2.0)])
+    hits, st, ref = _compare_search(oracle, data, 16, -2.79e-6, 18.25, 3.0, 0.0, 6.0,
+                                    tuning=tuning)
+    np.testing.assert_array_equal(st["top_sums"], ref.top_sums)
+    pos = ref.top_sums > 0
+    np.testing.assert_array_equal(st["top_blocks"][pos], ref.top_blocks[pos])
+    np.testing.assert_array_equal(st["top_offsets"][pos], ref.top_offsets[pos])
+    assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
+
+
+def test_device_stats_mode(gpu, oracle):
+    T, N = 16, 1 << 18
+    rng = np.random.default_rng(5)
+    tones = [(rng.uniform(0, N), rng.uniform(-3, 3), rng.uniform(2, 5)) for _ in range(30)]
+    data = _waterfall(T, N, 5, tones)
+    hits, st, ref = _compare_search(oracle, data, 16, -2.79e-6, 18.25, 2.0, 0.0, 6.0,
+                                    stats_mode=sc.SC_STATS_DEVICE)
+    assert st["median"] == ref.median  # exact rank statistic
+    assert abs(st["std"] - ref.std) <= 1e-5 * ref.std
+    med64, _, std64 = oracle.stats_fp64(data)
+    assert abs(st["std"] - std64) <= 1e-6 * std64  # FP64 trimmed moments
+    got = {(h.index, h.drift_steps) for h in hits}
+    want = {(h["index"], h["drift_steps"]) for h in ref.hits}
+    # Any difference must be a candidate at the threshold: its top sum lies
+    # between the two thresholds.
+    thr_dev = np.float32(6.0 * st["std"] + st["median"])
+    thr_ref = np.float32(6.0 * ref.std + ref.median)
+    lo, hi = min(thr_dev, thr_ref), max(thr_dev, thr_ref)
+    for idx, _ in got ^ want:
+        assert lo <= ref.top_sums[idx] <= hi, idx
+    assert len(got ^ want) <= 2
+    for h in hits:
+        snr64 = (oracle.path_sum_fp64(data, h.index, *divmod_drift(h.drift_steps, 15)) -
+                 med64) / std64
+        assert abs(h.snr - snr64) <= 1e-4 * max(1.0, abs(snr64))
+
+
+def divmod_drift(drift_steps, drift_timesteps):
+    b = drift_steps // drift_timesteps
+    return b, drift_steps - b * drift_timesteps
+
+
+def test_stale_blocks_quirk(gpu, oracle):
+    # dedoppler.cu:205 resets only the sums: channels with no positive path keep
+    # the previous search's block/offset.
+    T, N = 16, 5000
+    data1 = _waterfall(T, N, 1)
+    data2 = -_waterfall(T, N, 2)
+    dd = sc.Dedopplerer(T, N, -2.79e-6, 18.25)
+    meta = sc.FilterbankMetadata(coarse_channel_size=N)
+    dd.search(sc.FilterbankBuffer(T, N, data=_to_gpu(data1)), meta, -1, 0, 1.0, 0.0, 5.0, [])
+    s1 = dd.state()
+    dd.search(sc.FilterbankBuffer(T, N, data=_to_gpu(data2)), meta, -1, 0, 1.0, 0.0, 5.0, [])
+    s2 = dd.state()
+    assert np.all(s2["top_sums"] == 0)
+    np.testing.assert_array_equal(s2["top_blocks"], s1["top_blocks"])
+    np.testing.assert_array_equal(s2["top_offsets"], s1["top_offsets"])
+
+
+def test_config1_full_size_parity(gpu, oracle):
+    # One coarse channel at the benchmark's CPU-runnable size: 16 x 2^20, 8 tones,
+    # max drift 4 Hz/s, SNR threshold 10.
+    T, N = 16, 1 << 20
+    foff, tsamp = -2.79e-6, 18.25
+    torch = _torch()
+    rng = np.random.default_rng(1)
+    tones = []
+    for _ in range(8):
+        drift = rng.uniform(0.2, 4.0) * rng.choice([-1, 1])
+        slope = drift * tsamp / (foff * 1e6)
+        snr = rng.uniform(10, 100)
+        tones.append((float(rng.uniform(2000, N - 2000)), float(slope), float(snr / 4.0)))
+    x = torch.empty((T, N), dtype=torch.float32, device="cuda")
+    sc.gen_waterfall(x, T, seed=12345, tones=tones)
+    data = x.cpu().numpy()
+    hits, st, ref = _compare_search(oracle, data, T, foff, tsamp, 4.0, 0.0, 10.0)
+    np.testing.assert_array_equal(st["top_sums"], ref.top_sums)
+    np.testing.assert_array_equal(st["column_sums"], ref.column_sums)
+    assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
+    assert len(hits) >= 6
+
+
+def test_add_incoherent_power(gpu, oracle):
+    T, N = 16, 20000
+    data = _waterfall(T, N, 8, [(500.0, 1.0, 5.0), (9000.0, -1.5, 5.0)])
+    inco = _waterfall(T, N, 9)
+    dd = sc.Dedopplerer(T, N, -2.79e-6, 18.25)
+    hits = []
+    dd.search(sc.FilterbankBuffer(T, N, data=_to_gpu(data)), sc.FilterbankMetadata(), -1, 0,
+              2.0, 0.0, 5.0, hits)
+    assert hits
+    dd.addIncoherentPower(sc.FilterbankBuffer(T, N, data=_to_gpu(inco)), hits)
+    assert [h.drift_steps for h in hits] == sorted(h.drift_steps for h in hits)
+    for h in hits:
+        b = int(math.floor(np.float32(h.drift_steps) / np.float32(15)))
+        p = h.drift_steps - b * 15
+        want = oracle.basic_taylor_tree(inco, b)[p, h.index]
+        assert h.incoherent_power == want
+
+
+def test_max_drift_zero_is_an_error(gpu):
+    dd = sc.Dedopplerer(16, 1000, 1.0, 1.0)
+    buf = sc.FilterbankBuffer(16, 1000)
+    with pytest.raises(sc.SeticoreError):
+        dd.search(buf, sc.FilterbankMetadata(), -1, 0, 0.0, 0.0, 10.0, [])
+
+
+def test_shape_mismatch_is_an_error(gpu):
+    dd = sc.Dedopplerer(16, 1000, 1.0, 1.0)
+    with pytest.raises(sc.SeticoreError):
+        dd.search(sc.FilterbankBuffer(8, 1000), sc.FilterbankMetadata(), -1, 0, 1.0, 0.0, 10.0,
+                  [])
