@@ -106,6 +106,7 @@ def main():
     ap.add_argument("--cpu-sample-cc", type=int, default=2)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--group", type=int, default=0)
+    ap.add_argument("--prefetch", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -123,7 +124,7 @@ def main():
     dd = sc.Dedopplerer(T, N, FOFF, TSAMP, False)
     dd.set_stats_mode(sc.SC_STATS_DEVICE)
     if args.lanes:
-        dd.set_tuning(args.lanes, args.group)
+        dd.set_tuning(args.lanes, args.group, args.prefetch)
     meta = sc.FilterbankMetadata(fch1=8419.0, foff=FOFF, coarse_channel_size=N)
     bufs = [sc.FilterbankBuffer(T, N, data=data[cc]) for cc in range(num_cc)]
 

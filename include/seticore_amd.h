@@ -89,11 +89,13 @@ sc_status sc_dd_memory_usage(const sc_dd* dd, size_t* bytes);
 sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
 
 /* Shape of the fused tree kernel used when rounded_timesteps <= 16:
- * channels held per lane (4 or 8) and drift blocks per LDS window (2..8).
- * Returns SC_ERR_UNSUPPORTED for shapes that are not compiled. Results do not
- * depend on the shape; only speed does. No counterpart in the reference (its
- * tile shape is fixed, taylor.cu:170-189); used by the autotuner. */
-sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group);
+ * channels held per lane (4 or 8), drift blocks per LDS window (a multiple
+ * of channels_per_lane) and whether the next window is prefetched into
+ * registers (0/1). Returns SC_ERR_UNSUPPORTED for shapes that are not
+ * compiled. Results do not depend on the shape; only speed does. No
+ * counterpart in the reference (its tile shape is fixed, taylor.cu:170-189);
+ * used by the autotuner. */
+sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, int prefetch);
 
 /* Kernel timing for measurement (no counterpart in the reference, which only
  * times whole runs, taylor_benchmark.cu:22-40): when enabled, each search

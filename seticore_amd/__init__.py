@@ -88,7 +88,7 @@ def lib():
         "sc_dd_destroy": [vp],
         "sc_dd_memory_usage": [vp, P(sz)],
         "sc_dd_set_stats_mode": [vp, i],
-        "sc_dd_set_tuning": [vp, i, i],
+        "sc_dd_set_tuning": [vp, i, i, i],
         "sc_dd_set_timing": [vp, i],
         "sc_dd_kernel_time": [vp, P(d), P(lng)],
         "sc_dd_search": [vp, vp, i, i, i, lng, P(ScMeta), i, i, d, d, d, P(ScHit), sz,
@@ -314,8 +314,9 @@ class Dedopplerer:
     def set_stats_mode(self, mode):
         check(lib().sc_dd_set_stats_mode(self._h, int(mode)))
 
-    def set_tuning(self, channels_per_lane, drift_group):
-        check(lib().sc_dd_set_tuning(self._h, int(channels_per_lane), int(drift_group)))
+    def set_tuning(self, channels_per_lane, drift_group, prefetch=0):
+        check(lib().sc_dd_set_tuning(self._h, int(channels_per_lane), int(drift_group),
+                                     int(prefetch)))
 
     def set_timing(self, enable):
         check(lib().sc_dd_set_timing(self._h, int(bool(enable))))

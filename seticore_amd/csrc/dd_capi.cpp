@@ -57,9 +57,10 @@ size_t dtype_size(int dtype) { return dtype == SC_F32 ? 4 : 2; }
 
 // Default fused-kernel shape per T_r: channels per lane and drift blocks per
 // LDS window.
-void default_tuning(int T, int* lanes, int* group) {
+void default_tuning(int T, int* lanes, int* group, int* prefetch) {
   *lanes = (T == 16) ? 4 : 8;
-  *group = (T == 16) ? 4 : 8;
+  *group = (T == 16) ? 8 : 16;
+  *prefetch = 0;
 }
 
 }  // namespace
@@ -73,8 +74,9 @@ struct sc_dd {
   int drift_timesteps = 0;
   double resolution = 0;
   int stats_mode = SC_STATS_HOST_EXACT;
-  int lanes = 4;  // fused kernel: channels per lane
-  int group = 4;  // fused kernel: drift blocks per LDS window
+  int lanes = 4;     // fused kernel: channels per lane
+  int group = 8;     // fused kernel: drift blocks per LDS window
+  int prefetch = 0;  // fused kernel: register prefetch of the next window
   int device = 0;
 
   // Device state.
@@ -84,7 +86,8 @@ struct sc_dd {
   int* top_blk = nullptr;
   int* top_off = nullptr;
   void* stats_scratch = nullptr;
-  float* stats_out = nullptr;  // {median, std}
+  unsigned* stats_hist = nullptr;  // level-1 and level-2 histograms
+  float* stats_out = nullptr;  // {median, std, mean, count}
   int4* cand = nullptr;        // per-window candidates
   size_t cand_capacity = 0;
   float* tree_a = nullptr;     // T_r*N scratch for the staged path and tree API
@@ -143,6 +146,7 @@ void release(sc_dd* dd) {
   hipFree(dd->top_blk);
   hipFree(dd->top_off);
   hipFree(dd->stats_scratch);
+  hipFree(dd->stats_hist);
   hipFree(dd->stats_out);
   hipFree(dd->cand);
   hipFree(dd->tree_a);
@@ -330,7 +334,7 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
   dd->drift_timesteps = rounded - 1;
   // dedoppler.cu:87
   dd->resolution = 1e6 * foff_mhz / (dd->drift_timesteps * tsamp_s);
-  default_tuning(rounded, &dd->lanes, &dd->group);
+  default_tuning(rounded, &dd->lanes, &dd->group, &dd->prefetch);
   hipGetDevice(&dd->device);
 
   const size_t N = num_channels;
@@ -343,6 +347,8 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
   if (s == SC_OK)
     s = dev_alloc(dd, &dd->stats_scratch, sc::device_stats_scratch_bytes(num_channels),
                   "stats scratch");
+  if (s == SC_OK)
+    s = dev_alloc(dd, (void**)&dd->stats_hist, sc::device_stats_hist_bytes(), "stats hist");
   if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->stats_out, 4 * sizeof(float), "stats");
   // At most one candidate per window and window >= 2 (max_drift > 0).
   dd->cand_capacity = (N + 1) / 2 + 1;
@@ -394,13 +400,14 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode) {
   return SC_OK;
 }
 
-sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group) {
+sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, int prefetch) {
   if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
   if (dd->rounded > 16) return fail(SC_ERR_UNSUPPORTED, "no fused kernel above 16 time steps");
-  if (!sc::fused_supported(dd->rounded, channels_per_lane, drift_group))
-    return fail(SC_ERR_UNSUPPORTED, "no fused kernel for this (lanes, group) shape");
+  if (!sc::fused_supported(dd->rounded, channels_per_lane, drift_group, prefetch))
+    return fail(SC_ERR_UNSUPPORTED, "no fused kernel for this (lanes, group, prefetch) shape");
   dd->lanes = channels_per_lane;
   dd->group = drift_group;
+  dd->prefetch = prefetch;
   return SC_OK;
 }
 
@@ -426,13 +433,12 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
   dd->state_valid = false;
 
   // dedoppler.cu:205 (the reference zeroes the float sums; the key 0 means "no
-  // positive path").
-  SC_HIP(sc::launch_fill_u64(dd->top_key, N, 0ull, stream), "reset top keys");
+  // positive path"), plus the statistics histograms.
+  SC_HIP(sc::launch_reset(dd->top_key, N, dd->stats_hist, stream), "reset");
   if (dd->timing) SC_HIP(hipEventRecord(dd->ev_start, stream), "timing event");
-  if (T <= 16 && sc::fused_supported(T, dd->lanes, dd->group)) {
+  if (T <= 16 && sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch)) {
     SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
-                                     dd->group,
-                                     dd->top_key, dd->colsum, stream),
+                                     dd->group, dd->prefetch, dd->top_key, dd->colsum, stream),
            "fused tree");
   } else {
     SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
@@ -444,11 +450,14 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
     }
   }
   if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
-  SC_HIP(sc::launch_unpack_top(dd->top_key, N, T, r.bmin, dd->top_sum, dd->top_blk, dd->top_off,
-                               stream),
-         "unpack top");
 
-  if (dd->stats_mode == SC_STATS_HOST_EXACT) {
+  // Tiny coarse channels take the host statistics: the device selection
+  // assumes 0 < ceil(.05N) <= floor(.95N) (the reference's own nth_element
+  // calls have empty ranges below a few channels).
+  if (dd->stats_mode == SC_STATS_HOST_EXACT || N < 4096) {
+    SC_HIP(sc::launch_unpack_top(dd->top_key, N, T, r.bmin, dd->top_sum, dd->top_blk,
+                                 dd->top_off, stream),
+           "unpack top");
     // dedoppler.cu:232-240
     SC_HIP(hipMemcpyAsync(dd->h_colsum, dd->colsum, N * sizeof(float), hipMemcpyDeviceToHost,
                           stream), "copy column sums");
@@ -465,10 +474,12 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
     dd->last_std = std_dev;
     host_merge(dd, r, meta, beam, coarse_channel, min_drift, snr_threshold, median, std_dev);
   } else {
-    SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_out, stream),
+    SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
+                                   dd->stats_out, stream),
            "device stats");
-    SC_HIP(sc::launch_window_merge(dd->top_sum, dd->top_blk, dd->top_off, N, r.window,
-                                   snr_threshold, dd->stats_out, dd->cand, stream),
+    SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
+                                   dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off,
+                                   dd->cand, stream),
            "window merge");
     const int nwin = (int)(((long)N + r.window - 1) / r.window);
     SC_HIP(hipMemcpyAsync(dd->h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost,

@@ -16,16 +16,16 @@ namespace sc {
 // 0 means "no path sum > 0" (dedoppler.cu:205).
 typedef unsigned long long top_key_t;
 
-// Fused tree + top-path tracking for T_r <= 16 (all drift blocks, one interior
-// launch plus one launch over the edge tiles). C = channels per lane,
-// G = drift blocks per LDS window. Writes column sums for every channel and
-// max-merges top keys.
+// Fused tree + top-path tracking for T_r <= 16 over all drift blocks in one
+// launch. C = channels per lane, G = drift blocks per LDS window, P = 1 to
+// prefetch the next window into registers. Writes column sums for every
+// channel and max-merges top keys.
 hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, int N,
-                                 int bmin, int nblocks, int C, int G, top_key_t* top_key,
-                                 float* colsum, hipStream_t stream);
+                                 int bmin, int nblocks, int C, int G, int P,
+                                 top_key_t* top_key, float* colsum, hipStream_t stream);
 
-// Whether launch_fused_tree_top has a kernel for (T, C, G).
-bool fused_supported(int T, int C, int G);
+// Whether launch_fused_tree_top has a kernel for (T, C, G, P).
+bool fused_supported(int T, int C, int G, int P);
 
 // Level-min(T,16) sums for every time block of one drift block:
 // out[(tb*Tb + p)*N + c], Tb = min(T, 16), tb < T/Tb.
@@ -44,7 +44,8 @@ hipError_t launch_top_update(const float* sums, int T, int N, int drift, int bmi
 hipError_t launch_colsum(const void* x, int dtype, long stride, int T, int N, float* colsum,
                          hipStream_t stream);
 
-hipError_t launch_fill_u64(top_key_t* p, long n, top_key_t value, hipStream_t stream);
+// Zeroes the top keys and the statistics histograms before a search.
+hipError_t launch_reset(top_key_t* key, long n, unsigned* hist, hipStream_t stream);
 
 // Unpacks keys into the reference's three top arrays; channels without a
 // positive path get top_sum 0 and keep their previous block/offset.
@@ -52,20 +53,20 @@ hipError_t launch_unpack_top(const top_key_t* key, int N, int T, int bmin, float
                              int* top_blk, int* top_off, hipStream_t stream);
 
 // ---- device statistics -----------------------------------------------------
-// Scratch layout (device): see DeviceStatsScratch in dd_kernels.hip.
 size_t device_stats_scratch_bytes(int N);
-// Computes median (rank N/2), and the trimmed mean/std over sorted ranks
-// [ceil(.05N), floor(.95N)] into stats_out[0..1] = {median, std} (float), plus
-// stats_d[0..2] = {mean, std, median} in double for diagnostics.
-hipError_t launch_device_stats(const float* colsum, int N, void* scratch, float* stats_out,
-                               hipStream_t stream);
+size_t device_stats_hist_bytes();
+// From the column sums (hist: zeroed histogram scratch): median (rank N/2) and
+// the trimmed mean/std over the sorted ranks [ceil(.05N), floor(.95N)] into
+// stats_out = {median, std, mean}.
+hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsigned* hist,
+                               float* stats_out, hipStream_t stream);
 
-// Per-window merge (dedoppler.cu:273-303). cand_out[i] = {index or -1, sum bits,
-// drift block, path offset} for window i.
-hipError_t launch_window_merge(const float* top_sum, const int* top_blk, const int* top_off,
-                               int N, int window, double snr_threshold,
-                               const float* stats /* {median, std} */, int4* cand_out,
-                               hipStream_t stream);
+// Per-window merge (dedoppler.cu:273-303) fused with unpacking the keys into
+// the three top arrays. cand_out[i] = {index or -1, sum bits, drift block,
+// path offset} for window i.
+hipError_t launch_window_merge(const top_key_t* key, int N, int T, int bmin, int window,
+                               double snr_threshold, const float* stats, float* top_sum,
+                               int* top_blk, int* top_off, int4* cand_out, hipStream_t stream);
 
 // Gathers path sums for (offset, channel) pairs from a [T][N] buffer.
 hipError_t launch_gather(const float* sums, int N, const int* offsets, const int* chans,

@@ -48,23 +48,32 @@ __device__ __forceinline__ float from_next_lane(float x) {
 // ---------------------------------------------------------------------------
 // Top-path keys.
 //
-// A key is an FP64 number whose value orders (path sum, -code) lexicographically:
-// the float path sum widened to double (exact; the low 29 mantissa bits are
-// zero) with kCodeMask - code placed in those low bits, where
-// code = (drift_block - min_drift_block) * T_r + path_offset. The max of keys
-// is therefore the largest sum and, among equal sums, the lowest
-// (drift block, offset) - the reference's strict ">" in ascending order
-// (dedoppler.cu:40,49,214). Keys of sums <= 0 never exceed kEmptyKey, the key
-// of (0, largest code), matching "only sums > 0 register" (dedoppler.cu:205).
-// As unsigned 64-bit integers positive keys order the same way, so partial
-// results from different workgroups merge with a 64-bit atomic max; 0 means
-// "no positive path".
+// A key packs (path sum, code) into 64 bits whose order is the lexicographic
+// order of (sum, -code): the float's bit pattern, sign-extended and shifted
+// left by 29, with kCodeMask - code in the low 29 bits, where
+// code = (drift_block - min_drift_block) * T_r + path_offset. Keys of positive
+// sums are non-negative integers below 2^60, and read as FP64 they are finite
+// non-negative numbers in the same order, so the running max is one v_max_f64;
+// keys of negative sums read as negative numbers. The max of keys is the
+// largest sum and, among equal sums, the lowest (drift block, offset) - the
+// reference's strict ">" in ascending order (dedoppler.cu:40,49,214). Keys of
+// sums <= 0 never exceed kEmptyKey, the key of (0, largest code), matching
+// "only sums > 0 register" (dedoppler.cu:205). Positive keys order the same way
+// as unsigned integers, so partial results merge with a 64-bit atomic max;
+// 0 means "no positive path". (A NaN sample would make a finite key: inputs
+// must be finite, as power always is.)
 constexpr unsigned long long kCodeMask = 0x1FFFFFFFull;
 constexpr unsigned long long kEmptyKey = kCodeMask;
 
 __device__ __forceinline__ double make_key(float s, unsigned low_bits) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong((double)s) | low_bits;
-  return __longlong_as_double((long long)u);
+  const int bits = __float_as_int(s);
+  const unsigned hi = (unsigned)(bits >> 3);            // v_ashrrev_i32
+  const unsigned lo = ((unsigned)bits << 29) | low_bits;  // v_lshl_or_b32
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ float key_sum(unsigned long long k) {
+  return __uint_as_float((unsigned)(k >> 29));
 }
 
 // ---------------------------------------------------------------------------
@@ -129,48 +138,73 @@ struct TileShape {
 // LDS window for G drift blocks starting at b0: row t holds global channels
 // [c_w + t*b0, c_w + t*b0 + C*S), stored de-interleaved by C (element k at
 // sub-row k % C, column k / C) so that every lane reads consecutive words
-// (no bank conflicts) and every read address is a per-row base plus a
-// compile-time offset.
+// (no bank conflicts). Drift block g = C*h + g' of the window reads element
+// C*lane + j + t*g = sub-row (j + t*g') % C, column lane + (j + t*g') / C + t*h:
+// with g' unrolled and the t*h term folded into a per-row base that advances
+// by t per h step, every read is base + a compile-time offset.
 template <int T, int C, int G>
 struct Window {
+  static_assert(G % C == 0, "drift blocks per window must be a multiple of C");
   // Sub-row length, rounded so that a row is a whole number of 64-lane loads.
   static constexpr int S0 = (kWave * C + (T - 1) * (G - 1) + C - 1) / C;
   static constexpr int kStep = kWave / C;
   static constexpr int S = (S0 + kStep - 1) / kStep * kStep;
   static constexpr int kRow = C * S;
+  static constexpr int kLoads = kRow / kWave;  // loads per lane per row
   static constexpr int kFloats = T * kRow;
 };
 
 typedef __attribute__((address_space(3))) float lds_float;
 
-// Loads the window rows. Rows whose span lies inside [0, N) take unguarded
-// loads (immediate offsets off one row pointer); rows crossing an edge clamp
-// the address and substitute 0 for out-of-range samples.
-template <int T, int C, int G, typename In>
-__device__ __forceinline__ void load_window(lds_float* win, const In* __restrict__ x,
-                                            long stride, int N, int c_w, int b0, int row0) {
+// Window rows are fetched into registers (fetch_window) and then written to
+// LDS (store_window), so a window costs one memory latency and the next
+// window's loads can be in flight while the current one is processed. Rows
+// whose span lies inside [0, N) take unguarded loads off one row pointer; rows
+// crossing an edge clamp the address and substitute 0 for out-of-range samples.
+template <int T, int C, int G, typename In, int L>
+__device__ __forceinline__ void fetch_window(float (&tmp)[T][L], const In* __restrict__ x,
+                                             long stride, int N, int c_w, int b0, int row0) {
   using Wn = Window<T, C, G>;
+  static_assert(L == Wn::kLoads, "register staging size");
   const int lane = threadIdx.x;
-  lds_float* dst0 = win + (lane % C) * Wn::S + lane / C;
 #pragma unroll
   for (int t = 0; t < T; ++t) {
     const int g0 = c_w + t * b0;
     const In* row = x + (long)(row0 + t) * stride;
-    lds_float* dst = dst0 + t * Wn::kRow;
     if (g0 >= 0 && g0 + Wn::kRow <= N) {
       const In* src = row + g0 + lane;
 #pragma unroll
-      for (int k0 = 0; k0 < Wn::kRow; k0 += kWave) dst[k0 / C] = to_f32<In>(src[k0]);
+      for (int i = 0; i < Wn::kLoads; ++i) tmp[t][i] = to_f32<In>(src[i * kWave]);
     } else {
 #pragma unroll
-      for (int k0 = 0; k0 < Wn::kRow; k0 += kWave) {
-        const int gc = g0 + k0 + lane;
+      for (int i = 0; i < Wn::kLoads; ++i) {
+        const int gc = g0 + i * kWave + lane;
         const bool ok = (unsigned)gc < (unsigned)N;
         const float val = to_f32<In>(row[ok ? gc : 0]);
-        dst[k0 / C] = ok ? val : 0.0f;
+        tmp[t][i] = ok ? val : 0.0f;
       }
     }
   }
+}
+
+template <int T, int C, int G, int L>
+__device__ __forceinline__ void store_window(lds_float* win, const float (&tmp)[T][L]) {
+  using Wn = Window<T, C, G>;
+  static_assert(L == Wn::kLoads, "register staging size");
+  const int lane = threadIdx.x;
+  lds_float* dst0 = win + (lane % C) * Wn::S + lane / C;
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+#pragma unroll
+    for (int i = 0; i < Wn::kLoads; ++i) dst0[t * Wn::kRow + i * (kWave / C)] = tmp[t][i];
+}
+
+template <int T, int C, int G, typename In>
+__device__ __forceinline__ void load_window(lds_float* win, const In* __restrict__ x,
+                                            long stride, int N, int c_w, int b0, int row0) {
+  float tmp[T][Window<T, C, G>::kLoads];
+  fetch_window<T, C, G, In>(tmp, x, stride, N, c_w, b0, row0);
+  store_window<T, C, G>(win, tmp);
 }
 
 // Per-row LDS base addresses for this lane. The empty asm hides their common
@@ -187,15 +221,15 @@ __device__ __forceinline__ void row_bases(lds_float* win, lds_float* (&rp)[T]) {
   }
 }
 
-// Reads drift block g of the window (the sheared tile) into registers.
-template <int T, int C, int G, int g>
+// Reads drift block g' (of the current h step) of the window into registers.
+template <int T, int C, int G, int gp>
 __device__ __forceinline__ void read_block(lds_float* const (&rp)[T], float (&v)[T][C]) {
   using Wn = Window<T, C, G>;
 #pragma unroll
   for (int t = 0; t < T; ++t)
 #pragma unroll
     for (int j = 0; j < C; ++j) {
-      const int k = j + t * g;
+      const int k = j + t * gp;
       v[t][j] = rp[t][(k % C) * Wn::S + k / C];
     }
 }
@@ -208,55 +242,65 @@ __device__ __forceinline__ bool block_interior(int c_w, int b, int N) {
   return (c_w + (T - 1) * b >= 0) && (c_w + Sh::kOut - 1 + (T - 1) * b + (T - 1) < N);
 }
 
-// One drift block: tree, optional edge masking (dedoppler.cu:40-46, including
-// the early return that skips every offset of a block whose first path ends
-// left of channel 0), then the key max in ascending (block, offset) order.
-// The interior kernel takes the blocks that need no masking, the edge kernel
-// the others; together they cover every (tile, block) exactly once.
-template <int T, int C, int G, bool EDGE, int g>
-__device__ __forceinline__ void fused_block(lds_float* const (&rp)[T], int ng, int b0, int c_w,
-                                            int N, unsigned low0, double (&best)[C]) {
-  if constexpr (g < G) {
-    if (g < ng) {
-      const int b = b0 + g;
-      if (block_interior<T, C>(c_w, b, N) != EDGE) {
-        float v[T][C];
-        read_block<T, C, G, g>(rp, v);
-        tree_regs<T, C>(v);
-        if constexpr (EDGE) {
-          const int lane = threadIdx.x;
+// One drift block: tree, edge masking when the block is not interior for this
+// tile (dedoppler.cu:40-46, including the early return that skips every offset
+// of a block whose first path ends left of channel 0), then the key max in
+// ascending (block, offset) order.
+template <int T, int C, int G, int gp>
+__device__ __forceinline__ void fused_block(lds_float* const (&rp)[T], int g, int ng, int b0,
+                                            int c_w, int N, unsigned low0, double (&best)[C]) {
+  if constexpr (gp < C) {
+    if (g + gp < ng) {
+      const int b = b0 + g + gp;
+      float v[T][C];
+      read_block<T, C, G, gp>(rp, v);
+      tree_regs<T, C>(v);
+      if (!block_interior<T, C>(c_w, b, N)) {
+        // Skipped paths become 0: their keys never exceed kEmptyKey.
+        const int lane = threadIdx.x;
 #pragma unroll
-          for (int j = 0; j < C; ++j) {
-            const int c = c_w + lane * C + j;
-            const int last0 = c + (T - 1) * b;
-            const int pmax = (c >= N || last0 < 0) ? 0 : min(T, N - last0);
+        for (int j = 0; j < C; ++j) {
+          const int c = c_w + lane * C + j;
+          const int last0 = c + (T - 1) * b;
+          const int pmax = (c >= N || last0 < 0) ? 0 : min(T, N - last0);
 #pragma unroll
-            for (int p = 0; p < T; ++p)
-              if (p >= pmax) v[p][j] = -INFINITY;
-          }
-        }
-#pragma unroll
-        for (int p = 0; p < T; ++p) {
-          const unsigned low = low0 - (unsigned)(g * T + p);
-#pragma unroll
-          for (int j = 0; j < C; ++j) best[j] = fmax(best[j], make_key(v[p][j], low));
+          for (int p = 0; p < T; ++p)
+            if (p >= pmax) v[p][j] = 0.0f;
         }
       }
+      const unsigned lowg = low0 - (unsigned)((g + gp) * T);
+#pragma unroll
+      for (int p = 0; p < T; ++p) {
+        const unsigned low = lowg - (unsigned)p;
+#pragma unroll
+        for (int j = 0; j < C; ++j) best[j] = fmax(best[j], make_key(v[p][j], low));
+      }
     }
-    fused_block<T, C, G, EDGE, g + 1>(rp, ng, b0, c_w, N, low0, best);
+    fused_block<T, C, G, gp + 1>(rp, g, ng, b0, c_w, N, low0, best);
   }
 }
 
-// Grid: one 64-lane workgroup per (channel tile, drift group). Work items are
-// tile-major and dealt so that each XCD (workgroups go round-robin to the 8
-// XCDs) sweeps a contiguous range of tiles: a tile's drift groups share their
-// input rows in that XCD's L2. The interior kernel (EDGE=false) covers all
-// tiles and also writes the column sums; the edge kernel covers the tiles
-// [0, lo_tiles) and [hi_tile0, ntiles), where some drift blocks need masking.
-template <int T, int C, int G, bool EDGE, typename In>
+// Order-preserving map of a float onto uint32 (used by the statistics).
+__device__ __forceinline__ unsigned ord_key(float f) {
+  const unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_val(unsigned k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Grid: one 64-lane workgroup per work item = (channel tile, chunk of
+// consecutive drift groups of G blocks). The workgroup walks its chunk window
+// by window and keeps the running key max in registers across the chunk; with
+// PF it fetches the next window's rows into registers while it runs the
+// current window's blocks out of LDS (without PF a chunk is one window). Items are tile-major and dealt so that each XCD
+// (workgroups go round-robin to the 8 XCDs) sweeps a contiguous range of
+// tiles. The workgroup whose chunk holds drift block 0 also writes the tile's
+// column sums in the reference's sequential order (dedoppler.cu:68-71).
+template <int T, int C, int G, bool PF, typename In>
 __global__ void __launch_bounds__(kWave)
     fused_tree_top_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
-                          int ngroups, int lo_tiles, int hi_tile0, int total_items,
+                          int ngroups, int chunks, int groups_per_chunk, int total_items,
                           unsigned long long* __restrict__ top_key,
                           float* __restrict__ colsum) {
   using Sh = TileShape<T, C>;
@@ -267,17 +311,18 @@ __global__ void __launch_bounds__(kWave)
   const int per_xcd = gridDim.x / 8;
   const int item = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (item >= total_items) return;
-  int tile = item / ngroups;
-  const int grp = item - tile * ngroups;
-  if (EDGE && tile >= lo_tiles) tile = hi_tile0 + (tile - lo_tiles);
+  const int tile = item / chunks;
+  const int chunk = item - tile * chunks;
   const int c_w = tile * Sh::kOut;
-  const int b0 = bmin + grp * G;
-  const int ng = min(G, nblocks - grp * G);
+  const int g_begin = chunk * groups_per_chunk;
+  const int g_end = min(ngroups, g_begin + groups_per_chunk);
   const int lane = threadIdx.x;
 
-  if (!EDGE && b0 <= 0 && 0 < b0 + ng && lane < Sh::kValidLanes) {
-    // Column sums in the reference's sequential float order (dedoppler.cu:68-71),
-    // computed once per tile by the workgroup whose drift group holds block 0.
+  float pf[T][Wn::kLoads];
+  fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + g_begin * G, 0);
+  const int b_first = bmin + g_begin * G;
+  const int b_last = min(bmin + g_end * G, bmin + nblocks) - 1;
+  if (b_first <= 0 && 0 <= b_last && lane < Sh::kValidLanes) {
 #pragma unroll
     for (int j = 0; j < C; ++j) {
       const int c = c_w + lane * C + j;
@@ -289,17 +334,34 @@ __global__ void __launch_bounds__(kWave)
       }
     }
   }
-
-  load_window<T, C, G, In>(win, x, stride, N, c_w, b0, 0);
+  store_window<T, C, G>(win, pf);
   __syncthreads();
 
-  lds_float* rp[T];
-  row_bases<T, C, G>(win, rp);
   double best[C];
 #pragma unroll
   for (int j = 0; j < C; ++j) best[j] = __longlong_as_double((long long)kEmptyKey);
-  const unsigned low0 = (unsigned)kCodeMask - (unsigned)(grp * G * T);
-  fused_block<T, C, G, EDGE, 0>(rp, ng, b0, c_w, N, low0, best);
+
+  for (int grp = g_begin; grp < g_end; ++grp) {
+    // Without PF every work item is one drift group (the launcher makes
+    // groups_per_chunk 1), so there is no next window to fetch.
+    const bool more = PF && grp + 1 < g_end;
+    if (more) fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + (grp + 1) * G, 0);
+    const int b0 = bmin + grp * G;
+    const int ng = min(G, nblocks - grp * G);
+    const unsigned low0 = (unsigned)kCodeMask - (unsigned)(grp * G * T);
+    lds_float* rp[T];
+    row_bases<T, C, G>(win, rp);
+    for (int g = 0; g < ng; g += C) {
+      fused_block<T, C, G, 0>(rp, g, ng, b0, c_w, N, low0, best);
+#pragma unroll
+      for (int t = 1; t < T; ++t) rp[t] += t;  // next h step: rows shift by t columns
+    }
+    if (more) {
+      __syncthreads();
+      store_window<T, C, G>(win, pf);
+      __syncthreads();
+    }
+  }
 
   if (lane < Sh::kValidLanes) {
 #pragma unroll
@@ -320,17 +382,17 @@ __global__ void __launch_bounds__(kWave)
     tree_blocks_kernel(const In* __restrict__ x, long stride, int N, int drift,
                        float* __restrict__ out) {
   using Sh = TileShape<T, C>;
-  using Wn = Window<T, C, 1>;
+  using Wn = Window<T, C, C>;
   __shared__ float win_storage[Wn::kFloats];
   lds_float* win = (lds_float*)win_storage;
   const int c_w = blockIdx.x * Sh::kOut;
   const int tb = blockIdx.y;
-  load_window<T, C, 1, In>(win, x, stride, N, c_w, drift, tb * T);
+  load_window<T, C, C, In>(win, x, stride, N, c_w, drift, tb * T);
   __syncthreads();
   lds_float* rp[T];
-  row_bases<T, C, 1>(win, rp);
+  row_bases<T, C, C>(win, rp);
   float v[T][C];
-  read_block<T, C, 1, 0>(rp, v);
+  read_block<T, C, C, 0>(rp, v);
   tree_regs<T, C>(v);
   const int lane = threadIdx.x;
   if (lane < Sh::kValidLanes) {
@@ -395,10 +457,16 @@ __global__ void colsum_kernel(const In* __restrict__ x, long stride, int T, int 
   colsum[c] = s;
 }
 
-__global__ void fill_u64_kernel(top_key_t* p, long n, top_key_t value) {
+// Clears the top keys and the statistics histograms before a search (a fill
+// kernel rather than a memset, so the sequence can be captured in a graph).
+__global__ void reset_kernel(top_key_t* __restrict__ key, long n, unsigned* __restrict__ hist,
+                             long nh) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x)
-    p[i] = value;
+    key[i] = 0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nh;
+       i += (long)gridDim.x * blockDim.x)
+    hist[i] = 0;
 }
 
 __global__ void unpack_top_kernel(const top_key_t* __restrict__ key, int N, int T, int bmin,
@@ -411,230 +479,309 @@ __global__ void unpack_top_kernel(const top_key_t* __restrict__ key, int N, int 
     top_sum[c] = 0.0f;  // block/offset keep their previous values (dedoppler.cu:205)
   } else {
     const unsigned code = (unsigned)(kCodeMask - (k & kCodeMask));
-    top_sum[c] = (float)__longlong_as_double((long long)(k & ~kCodeMask));
+    top_sum[c] = key_sum(k);
     top_blk[c] = (int)(code / (unsigned)T) + bmin;
     top_off[c] = (int)(code % (unsigned)T);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Device statistics: exact order statistics by 4-pass 8-bit radix select, and
-// FP64 trimmed moments reduced in a fixed order (deterministic).
+// Device statistics (replacing the host's nth_element, dedoppler.cu:241-262):
+// exact order statistics at ranks mid = N/2, first = ceil(.05N) and
+// last = floor(.95N) by a two-level 16-bit radix select, and the trimmed
+// mean/deviation over the sorted ranks [first, last] in FP64.
+//   level 1: histogram of the top 16 bits of each column sum's order key
+//            (built where the column sums are computed);
+//   select:  one workgroup finds the level-1 bin of each rank;
+//   level 2: histograms of the low 16 bits inside those three bins, plus
+//            FP64 sums (relative to a shift K inside the median bin) of the
+//            values strictly between the first and last bins;
+//   final:   one workgroup resolves the exact values, adds the boundary
+//            bins' contributions from their level-2 histograms (every level-2
+//            bin is one exact float), and forms
+//              m   = float(sum) / count          (as the reference forms it)
+//              std = sqrt(sum (x - m)^2 / count) (FP64).
+// All reductions run in a fixed order, so results are deterministic.
 
-struct StatsState {
-  unsigned hist[3][256];
-  unsigned prefix[3];
-  unsigned rank[3];
-  double mean_sum;  // FP64 sum over the trimmed ranks
-  float mean_f;     // float(sum) / count, as the reference forms its mean
-  int k_lo, k_hi;   // how many copies of the boundary values are in the set
-  int pad;
-};
-
+constexpr int kHistBins = 65536;
 constexpr int kStatsBlocks = 256;
 constexpr int kStatsThreads = 256;
 
-struct StatsPartials {
-  double s[kStatsBlocks];
-  unsigned n_in[kStatsBlocks], lt_lo[kStatsBlocks], eq_lo[kStatsBlocks], lt_hi[kStatsBlocks],
-      eq_hi[kStatsBlocks];
+struct StatsState {
+  int bin[3];        // level-1 bins of ranks mid, first, last
+  int rank_in[3];    // residual rank inside that bin
+  int below[3];      // number of values in lower level-1 bins
+  float shift;       // K: the median bin's smallest value
+  int pad;
+  double part_s1[kStatsBlocks], part_s2[kStatsBlocks];
+  unsigned part_n[kStatsBlocks];
 };
 
-__device__ __forceinline__ unsigned ord_key(float f) {
-  const unsigned u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float ord_val(unsigned k) {
-  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
-}
+// Histogram buffers: hist_hi[65536] then hist_lo[3][65536].
+size_t stats_hist_words() { return (size_t)4 * kHistBins; }
 
-__global__ void stats_init_kernel(StatsState* st, int N) {
-  const int i = threadIdx.x;
-  for (int r = 0; r < 3; ++r) st->hist[r][i] = 0;
-  if (i == 0) {
-    const int mid = N / 2;
-    const int first = (int)ceil(0.05 * N);
-    const int last = (int)floor(0.95 * N);
-    st->rank[0] = mid;
-    st->rank[1] = first;
-    st->rank[2] = last;
-    for (int r = 0; r < 3; ++r) st->prefix[r] = 0;
+__device__ __forceinline__ int rank_first(int N) { return (int)ceil(0.05 * N); }
+__device__ __forceinline__ int rank_last(int N) { return (int)floor(0.95 * N); }
+
+// Finds, for each of three ranks, the bin of a 65536-bin histogram holding it.
+// One workgroup of 1024 threads, 64 bins each.
+__device__ void select_bins(const unsigned* __restrict__ hist, const int (&want)[3],
+                            int (&bin)[3], int (&rank_in)[3], int (&below)[3]) {
+  __shared__ unsigned scan[1024];
+  __shared__ int found[3][3];
+  const int tid = threadIdx.x;
+  if (tid < 9) found[tid / 3][tid % 3] = 0;
+  unsigned h[64];
+  unsigned local = 0;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) {
+    h[i] = hist[tid * 64 + i];
+    local += h[i];
   }
-}
-
-__global__ void __launch_bounds__(kStatsThreads)
-    radix_hist_kernel(const float* __restrict__ v, int N, int shift, StatsState* st) {
-  __shared__ unsigned h[3][256];
-  for (int r = 0; r < 3; ++r) h[r][threadIdx.x] = 0;
+  scan[tid] = local;
   __syncthreads();
-  const unsigned hi_mask = (shift >= 24) ? 0u : (0xFFFFFFFFu << (shift + 8));
-  const unsigned p0 = st->prefix[0] & hi_mask, p1 = st->prefix[1] & hi_mask,
-                 p2 = st->prefix[2] & hi_mask;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    const unsigned k = ord_key(v[i]);
-    const unsigned d = (k >> shift) & 255u;
-    const unsigned hk = k & hi_mask;
-    if (hk == p0) atomicAdd(&h[0][d], 1u);
-    if (hk == p1) atomicAdd(&h[1][d], 1u);
-    if (hk == p2) atomicAdd(&h[2][d], 1u);
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+    const unsigned v = tid >= off ? scan[tid - off] : 0u;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
   }
-  __syncthreads();
-  for (int r = 0; r < 3; ++r)
-    if (h[r][threadIdx.x]) atomicAdd(&st->hist[r][threadIdx.x], h[r][threadIdx.x]);
-}
-
-__global__ void radix_select_kernel(StatsState* st, int shift) {
-  if (threadIdx.x == 0) {
-    for (int r = 0; r < 3; ++r) {
-      unsigned want = st->rank[r];
-      unsigned cum = 0;
-      int d = 0;
-      for (; d < 256; ++d) {
-        const unsigned c = st->hist[r][d];
-        if (cum + c > want) break;
-        cum += c;
+  const unsigned before = tid ? scan[tid - 1] : 0u;
+  for (int r = 0; r < 3; ++r) {
+    const unsigned w = (unsigned)want[r];
+    if (w >= before && w < scan[tid]) {
+      unsigned cum = before;
+      int fi = 0;
+      unsigned fcum = cum;
+      bool done = false;
+#pragma unroll
+      for (int i = 0; i < 64; ++i) {
+        if (!done && w < cum + h[i]) {
+          fi = i;
+          fcum = cum;
+          done = true;
+        }
+        cum += h[i];
       }
-      if (d > 255) d = 255;
-      st->prefix[r] |= ((unsigned)d << shift);
-      st->rank[r] = want - cum;
+      found[r][0] = tid * 64 + fi;
+      found[r][1] = (int)(w - fcum);
+      found[r][2] = (int)fcum;
     }
   }
   __syncthreads();
-  for (int r = 0; r < 3; ++r) st->hist[r][threadIdx.x] = 0;
+  for (int r = 0; r < 3; ++r) {
+    bin[r] = found[r][0];
+    rank_in[r] = found[r][1];
+    below[r] = found[r][2];
+  }
+  __syncthreads();
 }
 
-template <typename T>
-__device__ __forceinline__ T block_sum(T x, T* sh) {
+// Level-1 histogram built apart from the tree kernel: column sums cluster in a
+// few hundred top-16-bit bins, so each workgroup first counts its share in LDS
+// (two 16-bit counters per word, at most 65535 values per workgroup) and then
+// adds its non-zero bins to the global histogram.
+constexpr int kHist1Threads = 1024;
+constexpr int kHist1PerBlock = 32768;
+
+__global__ void __launch_bounds__(kHist1Threads)
+    stats_hist1_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist_hi) {
+  __shared__ unsigned h[kHistBins / 2];
+  for (int i = threadIdx.x; i < kHistBins / 2; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const int begin = blockIdx.x * kHist1PerBlock;
+  const int end = min(N, begin + kHist1PerBlock);
+  for (int i = begin + threadIdx.x; i < end; i += blockDim.x) {
+    const unsigned b = ord_key(v[i]) >> 16;
+    atomicAdd(&h[b >> 1], (b & 1u) ? 65536u : 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHistBins / 2; i += blockDim.x) {
+    const unsigned w = h[i];
+    if (w & 0xFFFFu) atomicAdd(&hist_hi[2 * i], w & 0xFFFFu);
+    if (w >> 16) atomicAdd(&hist_hi[2 * i + 1], w >> 16);
+  }
+}
+
+__global__ void __launch_bounds__(1024)
+    stats_select_kernel(const unsigned* __restrict__ hist_hi, int N, StatsState* st) {
+  const int want[3] = {N / 2, rank_first(N), rank_last(N)};
+  int bin[3], rank_in[3], below[3];
+  select_bins(hist_hi, want, bin, rank_in, below);
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < 3; ++r) {
+      st->bin[r] = bin[r];
+      st->rank_in[r] = rank_in[r];
+      st->below[r] = below[r];
+    }
+    st->shift = ord_val((unsigned)bin[0] << 16);
+  }
+}
+
+template <typename V>
+__device__ __forceinline__ V block_sum(V x, V* sh) {
   sh[threadIdx.x] = x;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  const T r = sh[0];
+  const V r = sh[0];
   __syncthreads();
   return r;
 }
 
 __global__ void __launch_bounds__(kStatsThreads)
-    moments1_kernel(const float* __restrict__ v, int N, const StatsState* st,
-                    StatsPartials* part) {
+    stats_level2_kernel(const float* __restrict__ v, int N, StatsState* st,
+                        unsigned* __restrict__ hist_lo) {
   __shared__ double shd[kStatsThreads];
   __shared__ unsigned shu[kStatsThreads];
-  const float lo = ord_val(st->prefix[1]);
-  const float hi = ord_val(st->prefix[2]);
-  double s = 0.0;
-  unsigned n_in = 0, lt_lo = 0, eq_lo = 0, lt_hi = 0, eq_hi = 0;
+  const int b_mid = st->bin[0], b_lo = st->bin[1], b_hi = st->bin[2];
+  const double K = (double)st->shift;
+  double s1 = 0.0, s2 = 0.0;
+  unsigned n = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
     const float x = v[i];
-    if (x > lo && x < hi) {
-      s += (double)x;
-      ++n_in;
+    const unsigned k = ord_key(x);
+    const int hi = (int)(k >> 16);
+    const unsigned lo = k & 0xFFFFu;
+    if (hi == b_mid) atomicAdd(&hist_lo[lo], 1u);
+    if (hi == b_lo) atomicAdd(&hist_lo[kHistBins + lo], 1u);
+    if (hi == b_hi) atomicAdd(&hist_lo[2 * kHistBins + lo], 1u);
+    if (hi > b_lo && hi < b_hi) {
+      const double d = __dsub_rn((double)x, K);
+      s1 = __dadd_rn(s1, d);
+      s2 = __dadd_rn(s2, __dmul_rn(d, d));
+      ++n;
     }
-    lt_lo += (x < lo);
-    eq_lo += (x == lo);
-    lt_hi += (x < hi);
-    eq_hi += (x == hi);
   }
-  s = block_sum<double>(s, shd);
-  n_in = block_sum<unsigned>(n_in, shu);
-  lt_lo = block_sum<unsigned>(lt_lo, shu);
-  eq_lo = block_sum<unsigned>(eq_lo, shu);
-  lt_hi = block_sum<unsigned>(lt_hi, shu);
-  eq_hi = block_sum<unsigned>(eq_hi, shu);
+  s1 = block_sum<double>(s1, shd);
+  s2 = block_sum<double>(s2, shd);
+  n = block_sum<unsigned>(n, shu);
   if (threadIdx.x == 0) {
-    part->s[blockIdx.x] = s;
-    part->n_in[blockIdx.x] = n_in;
-    part->lt_lo[blockIdx.x] = lt_lo;
-    part->eq_lo[blockIdx.x] = eq_lo;
-    part->lt_hi[blockIdx.x] = lt_hi;
-    part->eq_hi[blockIdx.x] = eq_hi;
+    st->part_s1[blockIdx.x] = s1;
+    st->part_s2[blockIdx.x] = s2;
+    st->part_n[blockIdx.x] = n;
   }
 }
 
-__global__ void moments1_final_kernel(StatsState* st, const StatsPartials* part, int N,
-                                      int nblocks) {
-  if (threadIdx.x != 0) return;
-  double s = 0.0;
-  unsigned n_in = 0, lt_lo = 0, eq_lo = 0, lt_hi = 0, eq_hi = 0;
-  for (int b = 0; b < nblocks; ++b) {
-    s += part->s[b];
-    n_in += part->n_in[b];
-    lt_lo += part->lt_lo[b];
-    eq_lo += part->eq_lo[b];
-    lt_hi += part->lt_hi[b];
-    eq_hi += part->eq_hi[b];
+// Adds to (n, s1, s2) the values of level-1 bin `bin` whose level-2 digit lies
+// in [d0, d1) (each digit is one exact float value), relative to K.
+__device__ void add_digit_range(const unsigned* __restrict__ h, int bin, int d0, int d1,
+                                double K, double* sh, unsigned* shu, unsigned& n_out,
+                                double& s1_out, double& s2_out) {
+  double s1 = 0.0, s2 = 0.0;
+  unsigned n = 0;
+  for (int d = d0 + (int)threadIdx.x; d < d1; d += blockDim.x) {
+    const unsigned c = h[d];
+    if (c) {
+      const double x = (double)ord_val(((unsigned)bin << 16) | (unsigned)d);
+      const double dx = __dsub_rn(x, K);
+      s1 = __dadd_rn(s1, __dmul_rn((double)c, dx));
+      s2 = __dadd_rn(s2, __dmul_rn((double)c, __dmul_rn(dx, dx)));
+      n += c;
+    }
   }
-  (void)n_in;
-  (void)eq_hi;
-  const int first = (int)ceil(0.05 * N);
-  const int last = (int)floor(0.95 * N);
-  const int cnt = last + 1 - first;
-  const float lo = ord_val(st->prefix[1]);
-  const float hi = ord_val(st->prefix[2]);
-  int k_lo, k_hi;
-  if (lo == hi) {
+  s1 = block_sum<double>(s1, sh);
+  s2 = block_sum<double>(s2, sh);
+  n = block_sum<unsigned>(n, shu);
+  n_out += n;
+  s1_out = __dadd_rn(s1_out, s1);
+  s2_out = __dadd_rn(s2_out, s2);
+}
+
+// Exact digit of a residual rank inside a level-2 histogram (all threads).
+__device__ void digit_of_rank(const unsigned* __restrict__ h, int rank_in, int& digit,
+                              int& below_in) {
+  const int want[3] = {rank_in, rank_in, rank_in};
+  int bin[3], r_in[3], below[3];
+  select_bins(h, want, bin, r_in, below);
+  digit = bin[0];
+  below_in = below[0];
+}
+
+__global__ void __launch_bounds__(1024)
+    stats_final_kernel(const StatsState* st, const unsigned* __restrict__ hist_lo, int N,
+                       float* __restrict__ stats_out) {
+  __shared__ double shd[1024];
+  __shared__ unsigned shu[1024];
+  const int first = rank_first(N), last = rank_last(N);
+  const long cnt = (long)last + 1 - first;
+  const double K = (double)st->shift;
+  int d_mid, d_lo, d_hi, bl_mid, bl_lo, bl_hi;
+  digit_of_rank(hist_lo, st->rank_in[0], d_mid, bl_mid);
+  digit_of_rank(hist_lo + kHistBins, st->rank_in[1], d_lo, bl_lo);
+  digit_of_rank(hist_lo + 2 * kHistBins, st->rank_in[2], d_hi, bl_hi);
+  const int b_lo = st->bin[1], b_hi = st->bin[2];
+  const float median = ord_val(((unsigned)st->bin[0] << 16) | (unsigned)d_mid);
+  const float v_lo = ord_val(((unsigned)b_lo << 16) | (unsigned)d_lo);
+  const float v_hi = ord_val(((unsigned)b_hi << 16) | (unsigned)d_hi);
+
+  // Values strictly between the first and last level-1 bins.
+  double s1 = 0.0, s2 = 0.0;
+  unsigned n = 0;
+  if (threadIdx.x == 0) {
+    for (int b = 0; b < kStatsBlocks; ++b) {
+      s1 = __dadd_rn(s1, st->part_s1[b]);
+      s2 = __dadd_rn(s2, st->part_s2[b]);
+      n += st->part_n[b];
+    }
+  }
+  // Boundary bins: values strictly inside (v_lo, v_hi) from the level-2
+  // histograms, then the copies of v_lo and v_hi that fall in [first, last].
+  const int below_lo = st->below[1] + bl_lo;  // values < v_lo
+  const int below_hi = st->below[2] + bl_hi;  // values < v_hi
+  const unsigned eq_lo = hist_lo[kHistBins + d_lo];
+  long k_lo, k_hi;
+  if (b_lo == b_hi) {
+    add_digit_range(hist_lo + kHistBins, b_lo, d_lo + 1, d_hi, K, shd, shu, n, s1, s2);
+  } else {
+    add_digit_range(hist_lo + kHistBins, b_lo, d_lo + 1, kHistBins, K, shd, shu, n, s1, s2);
+    add_digit_range(hist_lo + 2 * kHistBins, b_hi, 0, d_hi, K, shd, shu, n, s1, s2);
+  }
+  if (v_lo == v_hi) {
     k_lo = cnt;
     k_hi = 0;
   } else {
-    k_lo = min((int)(lt_lo + eq_lo), last + 1) - first;
-    k_hi = last + 1 - (int)lt_hi;
+    k_lo = min((long)below_lo + (long)eq_lo, (long)last + 1) - first;
+    k_hi = (long)last + 1 - below_hi;
   }
-  s = __dadd_rn(s, __dadd_rn(__dmul_rn((double)k_lo, (double)lo), __dmul_rn((double)k_hi, (double)hi)));
-  st->k_lo = k_lo;
-  st->k_hi = k_hi;
-  st->mean_sum = s;
-  const float sum_f = (float)s;
-  st->mean_f = sum_f / (float)cnt;
-}
-
-__global__ void __launch_bounds__(kStatsThreads)
-    moments2_kernel(const float* __restrict__ v, int N, const StatsState* st,
-                    StatsPartials* part) {
-  __shared__ double shd[kStatsThreads];
-  const float lo = ord_val(st->prefix[1]);
-  const float hi = ord_val(st->prefix[2]);
-  const double m = (double)st->mean_f;
-  double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    const float x = v[i];
-    if (x > lo && x < hi) {
-      const double d = __dsub_rn((double)x, m);
-      acc = __dadd_rn(acc, __dmul_rn(d, d));
-    }
+  if (threadIdx.x == 0) {
+    const double dl = __dsub_rn((double)v_lo, K), dh = __dsub_rn((double)v_hi, K);
+    s1 = __dadd_rn(s1, __dadd_rn(__dmul_rn((double)k_lo, dl), __dmul_rn((double)k_hi, dh)));
+    s2 = __dadd_rn(s2, __dadd_rn(__dmul_rn((double)k_lo, __dmul_rn(dl, dl)),
+                                 __dmul_rn((double)k_hi, __dmul_rn(dh, dh))));
+    // float sum = accumulate(...) in double; float m = sum / count.
+    const double total = __dadd_rn(__dmul_rn((double)cnt, K), s1);
+    const float sum_f = (float)total;
+    const float m = sum_f / (float)cnt;
+    const double delta = __dsub_rn((double)m, K);
+    // sum (x - m)^2 = sum (x-K)^2 - 2 delta sum (x-K) + cnt delta^2
+    const double acc = __dadd_rn(__dsub_rn(s2, __dmul_rn(2.0 * delta, s1)),
+                                 __dmul_rn((double)cnt, __dmul_rn(delta, delta)));
+    stats_out[0] = median;
+    stats_out[1] = (float)sqrt(fmax(acc, 0.0) / (double)cnt);
+    stats_out[2] = m;
+    stats_out[3] = (float)n;
   }
-  acc = block_sum<double>(acc, shd);
-  if (threadIdx.x == 0) part->s[blockIdx.x] = acc;
-}
-
-__global__ void moments2_final_kernel(const StatsState* st, const StatsPartials* part, int N,
-                                      int nblocks, float* stats_out) {
-  if (threadIdx.x != 0) return;
-  double acc = 0.0;
-  for (int b = 0; b < nblocks; ++b) acc = __dadd_rn(acc, part->s[b]);
-  const double m = (double)st->mean_f;
-  const float lo = ord_val(st->prefix[1]);
-  const float hi = ord_val(st->prefix[2]);
-  const double dl = __dsub_rn((double)lo, m), dh = __dsub_rn((double)hi, m);
-  acc = __dadd_rn(acc, __dadd_rn(__dmul_rn((double)st->k_lo, __dmul_rn(dl, dl)),
-                                 __dmul_rn((double)st->k_hi, __dmul_rn(dh, dh))));
-  const int first = (int)ceil(0.05 * N);
-  const int last = (int)floor(0.95 * N);
-  const int cnt = last + 1 - first;
-  stats_out[0] = ord_val(st->prefix[0]);
-  stats_out[1] = (float)sqrt(acc / (double)cnt);
 }
 
 // ---------------------------------------------------------------------------
-// Window merge (dedoppler.cu:273-303): one wavefront per window.
+// Window merge (dedoppler.cu:273-303), one workgroup per window, fused with the
+// unpacking of the keys into the reference's three top arrays (each
+// workgroup unpacks its own window; channels without a positive path keep
+// their previous block and offset, dedoppler.cu:205).
 __global__ void __launch_bounds__(256)
-    window_merge_kernel(const float* __restrict__ top, const int* __restrict__ blk,
-                        const int* __restrict__ off, int N, int window, int nwin,
+    window_merge_kernel(const top_key_t* __restrict__ key, int N, int T, int bmin, int window,
                         double snr_threshold, const float* __restrict__ stats,
-                        int4* __restrict__ out) {
-  const int wid = blockIdx.x * 4 + threadIdx.x / kWave;
-  const int lane = threadIdx.x % kWave;
-  if (wid >= nwin) return;
+                        float* __restrict__ top_sum, int* __restrict__ top_blk,
+                        int* __restrict__ top_off, int4* __restrict__ out) {
+  __shared__ float sv[4];
+  __shared__ int si[4];
+  __shared__ int any_larger;
+  const int wid = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave, wave = tid / kWave;
   const float median = stats[0];
   const float sd = stats[1];
   // float path_sum_threshold = snr_threshold * std_dev + median; (double math, no FMA)
@@ -643,8 +790,15 @@ __global__ void __launch_bounds__(256)
   const long hi = min(lo + (long)window, (long)N);
   float bv = thr;
   int bi = -1;
-  for (long f = lo + lane; f < hi; f += kWave) {
-    const float v = top[f];
+  for (long f = lo + tid; f < hi; f += blockDim.x) {
+    const top_key_t k = key[f];
+    const float v = k ? key_sum(k) : 0.0f;
+    if (k) {
+      const unsigned code = (unsigned)(kCodeMask - (k & kCodeMask));
+      top_blk[f] = (int)(code / (unsigned)T) + bmin;
+      top_off[f] = (int)(code % (unsigned)T);
+    }
+    top_sum[f] = v;
     if (v > bv) {
       bv = v;
       bi = (int)f;
@@ -659,20 +813,42 @@ __global__ void __launch_bounds__(256)
       bi = oi;
     }
   }
+  if (lane == 0) {
+    sv[wave] = bv;
+    si[wave] = bi;
+  }
+  if (tid == 0) any_larger = 0;
+  __syncthreads();
+  bv = sv[0];
+  bi = si[0];
+  for (int w = 1; w < 4; ++w) {
+    const float ov = sv[w];
+    const int oi = si[w];
+    if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) {
+      bv = ov;
+      bi = oi;
+    }
+  }
   if (bi < 0) {
-    if (lane == 0) out[wid] = make_int4(-1, 0, 0, 0);
+    if (tid == 0) out[wid] = make_int4(-1, 0, 0, 0);
     return;
   }
   const long s0 = max(0L, (long)bi - window + 1);
   const long s1 = min((long)N, (long)bi + window);
-  bool larger = false;
-  for (long f = s0 + lane; f < s1; f += kWave) larger |= (top[f] > bv);
-  larger = __any(larger);
-  if (lane == 0) {
-    if (larger)
+  int larger = 0;
+  for (long f = s0 + tid; f < s1; f += blockDim.x) {
+    const top_key_t k = key[f];
+    larger |= (k ? key_sum(k) : 0.0f) > bv;
+  }
+  if (larger) any_larger = 1;
+  __syncthreads();
+  if (tid == 0) {
+    if (any_larger) {
       out[wid] = make_int4(-1, 0, 0, 0);
-    else
-      out[wid] = make_int4(bi, __float_as_int(bv), blk[bi], off[bi]);
+    } else {
+      // The candidate lies in this window, so this workgroup unpacked it.
+      out[wid] = make_int4(bi, __float_as_int(bv), top_blk[bi], top_off[bi]);
+    }
   }
 }
 
@@ -809,58 +985,51 @@ inline int blocks_for(long n, int threads, int cap) {
   return (int)b;
 }
 
-template <int T, int C, int G, typename In>
+template <int T, int C, int G, bool PF, typename In>
 hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nblocks,
                               top_key_t* top_key, float* colsum, hipStream_t stream) {
   using Sh = TileShape<T, C>;
   const long ntiles = ((long)N + Sh::kOut - 1) / Sh::kOut;
   const long ngroups = ((long)nblocks + G - 1) / G;
-  // Edge tiles: some drift block has paths that the reference skips
-  // (dedoppler.cu:42-46) at the left (c + (T-1)*b < 0) or right (end >= N) end.
-  const long bmax = (long)bmin + nblocks - 1;
-  long lo = 0;
-  if (bmin < 0) lo = std::min(ntiles, ((long)(T - 1) * (-bmin) + Sh::kOut - 1) / Sh::kOut);
-  const long thr = (long)N - Sh::kOut + 1 - (long)(T - 1) * (bmax + 1);
-  long hi0 = thr <= 0 ? 0 : (thr + Sh::kOut - 1) / Sh::kOut;
-  hi0 = std::min(hi0, ntiles);
-  if (hi0 < lo) hi0 = lo;
-  const long edge_tiles = lo + (ntiles - hi0);
-  const long total = ntiles * ngroups;
+  // With PF: chunks of drift groups per tile, as long as the chip is still
+  // filled several times over (256 CUs x a few resident workgroups).
+  long chunks = ngroups;
+  if (PF) {
+    chunks = 1;
+    while (ntiles * chunks < 8192 && chunks < ngroups) ++chunks;
+  }
+  const long per_chunk = (ngroups + chunks - 1) / chunks;
+  chunks = (ngroups + per_chunk - 1) / per_chunk;
+  const long total = ntiles * chunks;
   if (total > (1L << 30)) return hipErrorInvalidValue;
   const int grid = (int)((total + 7) / 8 * 8);
-  hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, false, In>), dim3(grid), dim3(kWave), 0,
-                     stream, x, stride, N, bmin, nblocks, (int)ngroups, (int)lo, (int)hi0,
-                     (int)total, top_key, colsum);
-  if (edge_tiles > 0) {
-    const long etotal = edge_tiles * ngroups;
-    const int egrid = (int)((etotal + 7) / 8 * 8);
-    hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, true, In>), dim3(egrid), dim3(kWave), 0,
-                       stream, x, stride, N, bmin, nblocks, (int)ngroups, (int)lo, (int)hi0,
-                       (int)etotal, top_key, colsum);
-  }
+  hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, PF, In>), dim3(grid), dim3(kWave), 0,
+                     stream, x, stride, N, bmin, nblocks, (int)ngroups, (int)chunks,
+                     (int)per_chunk, (int)total, top_key, colsum);
   return hipGetLastError();
 }
 
 // (C, G) variants compiled for each T: C channels per lane, G drift blocks per
 // LDS window.
 #define SC_FUSED_VARIANTS(X) \
-  X(2, 8, 8)                 \
-  X(4, 8, 8)                 \
-  X(8, 8, 8)                 \
-  X(8, 4, 8)                 \
-  X(16, 4, 2)                \
-  X(16, 4, 4)                \
-  X(16, 4, 8)                \
-  X(16, 8, 2)                \
-  X(16, 8, 4)
+  X(2, 8, 16, 0)             \
+  X(4, 8, 16, 0)             \
+  X(8, 8, 16, 0)             \
+  X(8, 4, 16, 0)             \
+  X(16, 4, 4, 0)             \
+  X(16, 4, 8, 0)             \
+  X(16, 4, 16, 0)            \
+  X(16, 4, 4, 1)             \
+  X(16, 4, 8, 1)             \
+  X(16, 8, 8, 0)
 
 template <typename In>
 hipError_t fused_dispatch(const In* x, long stride, int T, int N, int bmin, int nblocks, int C,
-                          int G, top_key_t* top_key, float* colsum, hipStream_t stream) {
-#define SC_FUSED_CASE(TT, CC, GG)                                                      \
-  if (T == TT && C == CC && G == GG)                                                   \
-    return launch_fused_typed<TT, CC, GG, In>(x, stride, N, bmin, nblocks, top_key, colsum, \
-                                              stream);
+                          int G, int P, top_key_t* top_key, float* colsum, hipStream_t stream) {
+#define SC_FUSED_CASE(TT, CC, GG, PP)                                                     \
+  if (T == TT && C == CC && G == GG && P == PP)                                           \
+    return launch_fused_typed<TT, CC, GG, PP != 0, In>(x, stride, N, bmin, nblocks, top_key, \
+                                                       colsum, stream);
   SC_FUSED_VARIANTS(SC_FUSED_CASE)
 #undef SC_FUSED_CASE
   return hipErrorInvalidValue;
@@ -893,27 +1062,27 @@ hipError_t tree_blocks_dispatch(const In* x, long stride, int T, int N, int drif
 
 }  // namespace
 
-bool fused_supported(int T, int C, int G) {
-#define SC_FUSED_SUPPORTED(TT, CC, GG) \
-  if (T == TT && C == CC && G == GG) return true;
+bool fused_supported(int T, int C, int G, int P) {
+#define SC_FUSED_SUPPORTED(TT, CC, GG, PP) \
+  if (T == TT && C == CC && G == GG && P == PP) return true;
   SC_FUSED_VARIANTS(SC_FUSED_SUPPORTED)
 #undef SC_FUSED_SUPPORTED
   return false;
 }
 
 hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, int N, int bmin,
-                                 int nblocks, int C, int G, top_key_t* top_key, float* colsum,
-                                 hipStream_t stream) {
+                                 int nblocks, int C, int G, int P, top_key_t* top_key,
+                                 float* colsum, hipStream_t stream) {
   switch (dtype) {
     case 0:
-      return fused_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, C, G, top_key,
-                                   colsum, stream);
+      return fused_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, C, G, P,
+                                   top_key, colsum, stream);
     case 1:
-      return fused_dispatch<__half>((const __half*)x, stride, T, N, bmin, nblocks, C, G,
+      return fused_dispatch<__half>((const __half*)x, stride, T, N, bmin, nblocks, C, G, P,
                                     top_key, colsum, stream);
     case 2:
       return fused_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N, bmin,
-                                            nblocks, C, G, top_key, colsum, stream);
+                                            nblocks, C, G, P, top_key, colsum, stream);
   }
   return hipErrorInvalidValue;
 }
@@ -965,9 +1134,9 @@ hipError_t launch_colsum(const void* x, int dtype, long stride, int T, int N, fl
   return hipGetLastError();
 }
 
-hipError_t launch_fill_u64(top_key_t* p, long n, top_key_t value, hipStream_t stream) {
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, stream, p, n,
-                     value);
+hipError_t launch_reset(top_key_t* key, long n, unsigned* hist, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_kernel, dim3(blocks_for(n, 256, 2048)), dim3(256), 0, stream, key, n,
+                     hist, (long)stats_hist_words());
   return hipGetLastError();
 }
 
@@ -978,38 +1147,28 @@ hipError_t launch_unpack_top(const top_key_t* key, int N, int T, int bmin, float
   return hipGetLastError();
 }
 
-size_t device_stats_scratch_bytes(int /*N*/) {
-  return ((sizeof(StatsState) + 255) / 256) * 256 + sizeof(StatsPartials);
-}
+size_t device_stats_scratch_bytes(int /*N*/) { return sizeof(StatsState); }
+size_t device_stats_hist_bytes() { return sizeof(unsigned) * stats_hist_words(); }
 
-hipError_t launch_device_stats(const float* colsum, int N, void* scratch, float* stats_out,
-                               hipStream_t stream) {
+hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsigned* hist,
+                               float* stats_out, hipStream_t stream) {
   StatsState* st = (StatsState*)scratch;
-  StatsPartials* part =
-      (StatsPartials*)((char*)scratch + ((sizeof(StatsState) + 255) / 256) * 256);
-  const int nb = blocks_for(N, kStatsThreads, kStatsBlocks);
-  hipLaunchKernelGGL(stats_init_kernel, dim3(1), dim3(256), 0, stream, st, N);
-  for (int shift = 24; shift >= 0; shift -= 8) {
-    hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kStatsThreads), 0, stream, colsum, N,
-                       shift, st);
-    hipLaunchKernelGGL(radix_select_kernel, dim3(1), dim3(256), 0, stream, st, shift);
-  }
-  hipLaunchKernelGGL(moments1_kernel, dim3(nb), dim3(kStatsThreads), 0, stream, colsum, N, st,
-                     part);
-  hipLaunchKernelGGL(moments1_final_kernel, dim3(1), dim3(64), 0, stream, st, part, N, nb);
-  hipLaunchKernelGGL(moments2_kernel, dim3(nb), dim3(kStatsThreads), 0, stream, colsum, N, st,
-                     part);
-  hipLaunchKernelGGL(moments2_final_kernel, dim3(1), dim3(64), 0, stream, st, part, N, nb,
+  hipLaunchKernelGGL(stats_hist1_kernel, dim3((N + kHist1PerBlock - 1) / kHist1PerBlock),
+                     dim3(kHist1Threads), 0, stream, colsum, N, hist);
+  hipLaunchKernelGGL(stats_select_kernel, dim3(1), dim3(1024), 0, stream, hist, N, st);
+  hipLaunchKernelGGL(stats_level2_kernel, dim3(kStatsBlocks), dim3(kStatsThreads), 0, stream,
+                     colsum, N, st, hist + kHistBins);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(1024), 0, stream, st, hist + kHistBins, N,
                      stats_out);
   return hipGetLastError();
 }
 
-hipError_t launch_window_merge(const float* top_sum, const int* top_blk, const int* top_off,
-                               int N, int window, double snr_threshold, const float* stats,
-                               int4* cand_out, hipStream_t stream) {
+hipError_t launch_window_merge(const top_key_t* key, int N, int T, int bmin, int window,
+                               double snr_threshold, const float* stats, float* top_sum,
+                               int* top_blk, int* top_off, int4* cand_out, hipStream_t stream) {
   const int nwin = (int)(((long)N + window - 1) / window);
-  hipLaunchKernelGGL(window_merge_kernel, dim3((nwin + 3) / 4), dim3(256), 0, stream, top_sum,
-                     top_blk, top_off, N, window, nwin, snr_threshold, stats, cand_out);
+  hipLaunchKernelGGL(window_merge_kernel, dim3(nwin), dim3(256), 0, stream, key, N, T, bmin,
+                     window, snr_threshold, stats, top_sum, top_blk, top_off, cand_out);
   return hipGetLastError();
 }
 

@@ -146,7 +146,7 @@ def test_search_parity_host_exact(gpu, oracle, case):
         assert _hit_fields(a) == _hit_fields(b)
 
 
-@pytest.mark.parametrize("tuning", [(4, 2), (4, 4), (4, 8), (8, 2), (8, 4)])
+@pytest.mark.parametrize("tuning", [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1), (8, 8, 0)])
 def test_tunings_are_bit_identical(gpu, oracle, tuning):
     T, N = 16, 50000
     data = _waterfall(T, N, 21, [(1000.5, 1.3, 3.0), (30000.0, -2.2, 2.0)])
@@ -167,7 +167,9 @@ def test_device_stats_mode(gpu, oracle):
     hits, st, ref = _compare_search(oracle, data, 16, -2.79e-6, 18.25, 2.0, 0.0, 6.0,
                                     stats_mode=sc.SC_STATS_DEVICE)
     assert st["median"] == ref.median  # exact rank statistic
-    assert abs(st["std"] - ref.std) <= 1e-5 * ref.std
+    # The reference accumulates the squared deviations in float over ~0.9 N
+    # terms; its std carries ~1e-5..1e-4 relative rounding error.
+    assert abs(st["std"] - ref.std) <= 1e-4 * ref.std
     med64, _, std64 = oracle.stats_fp64(data)
     assert abs(st["std"] - std64) <= 1e-6 * std64  # FP64 trimmed moments
     got = {(h.index, h.drift_steps) for h in hits}
