@@ -97,6 +97,11 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
  * used by the autotuner. */
 sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, int prefetch);
 
+/* Whether device-statistics searches replay a cached hipGraph of their
+ * launch sequence (one graph per input pointer and search parameters, up to
+ * 256; captured on first use). Default on. No counterpart in the reference. */
+sc_status sc_dd_set_graphs(sc_dd* dd, int enable);
+
 /* Kernel timing for measurement (no counterpart in the reference, which only
  * times whole runs, taylor_benchmark.cu:22-40): when enabled, each search
  * records HIP events on its stream around the path-sum kernels (the tree and

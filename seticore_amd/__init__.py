@@ -52,12 +52,37 @@ class ScMeta(ctypes.Structure):
                 ("num_coherent_beams", ctypes.c_int)]
 
 
-class ScHit(ctypes.Structure):
+class DedopplerHit(ctypes.Structure):
+    """dedoppler_hit.h:11-66. The structure is the C ABI's sc_hit, so hits come
+    back from the library with one buffer copy."""
+
     _fields_ = [("frequency", ctypes.c_double), ("index", ctypes.c_int),
                 ("drift_steps", ctypes.c_int), ("drift_rate", ctypes.c_double),
                 ("snr", ctypes.c_float), ("coarse_channel", ctypes.c_int),
                 ("beam", ctypes.c_int), ("num_timesteps", ctypes.c_int),
                 ("power", ctypes.c_float), ("incoherent_power", ctypes.c_float)]
+
+    def lowIndex(self):
+        return min(self.index, self.index + self.drift_steps)
+
+    def highIndex(self):
+        return max(self.index, self.index + self.drift_steps)
+
+    def toString(self):
+        """dedoppler_hit.cpp:25-30 (fmt "{:.5f}" == printf "%.5f")."""
+        n = self.drift_steps
+        bins = "%d bin" % n if n in (1, -1) else "%d bins" % n
+        return "coarse channel = %d, index = %d, snr = %.5f, drift rate = %.5f (%s)" % (
+            self.coarse_channel, self.index, self.snr, self.drift_rate, bins)
+
+    def __repr__(self):
+        return "DedopplerHit(%s)" % self.toString()
+
+    def key(self):
+        return (self.index, self.drift_steps, self.coarse_channel, self.beam)
+
+
+ScHit = DedopplerHit
 
 
 class ScFileMeta(ctypes.Structure):
@@ -90,6 +115,7 @@ def lib():
         "sc_dd_set_stats_mode": [vp, i],
         "sc_dd_set_tuning": [vp, i, i, i],
         "sc_dd_set_timing": [vp, i],
+        "sc_dd_set_graphs": [vp, i],
         "sc_dd_kernel_time": [vp, P(d), P(lng)],
         "sc_dd_search": [vp, vp, i, i, i, lng, P(ScMeta), i, i, d, d, d, P(ScHit), sz,
                          P(sz), vp],
@@ -144,7 +170,7 @@ def _torch():
 def _stream_ptr(stream=None):
     torch = _torch()
     s = stream if stream is not None else torch.cuda.current_stream()
-    return ctypes.c_void_p(s.cuda_stream)
+    return s.cuda_stream
 
 
 # ---------------------------------------------------------------------------
@@ -181,8 +207,13 @@ class FilterbankMetadata:
         return 0 <= beam < len(self.source_names)
 
     def _sc_meta(self):
-        return ScMeta(self.fch1, self.foff, int(self.coarse_channel_size),
-                      len(self.source_names))
+        key = (self.fch1, self.foff, self.coarse_channel_size, len(self.source_names))
+        cached = getattr(self, "_meta_cache", None)
+        if cached is None or cached[0] != key:
+            cached = (key, ScMeta(self.fch1, self.foff, int(self.coarse_channel_size),
+                                  len(self.source_names)))
+            self._meta_cache = cached
+        return cached[1]
 
     def _sc_file_meta(self):
         return ScFileMeta(self.source_name.encode(), self.fch1, self.foff, self.tstart,
@@ -241,42 +272,6 @@ def makeNoisyBuffer(num_timesteps, num_channels):
     return buf
 
 
-class DedopplerHit:
-    """dedoppler_hit.h:11-66"""
-
-    __slots__ = ("frequency", "index", "drift_steps", "drift_rate", "snr", "coarse_channel",
-                 "beam", "num_timesteps", "power", "incoherent_power")
-
-    @classmethod
-    def _from_c(cls, h):
-        o = cls.__new__(cls)
-        for name, _ in ScHit._fields_:
-            setattr(o, name, getattr(h, name))
-        return o
-
-    def _to_c(self):
-        return ScHit(*[getattr(self, name) for name, _ in ScHit._fields_])
-
-    def lowIndex(self):
-        return min(self.index, self.index + self.drift_steps)
-
-    def highIndex(self):
-        return max(self.index, self.index + self.drift_steps)
-
-    def toString(self):
-        """dedoppler_hit.cpp:25-30 (fmt "{:.5f}" == printf "%.5f")."""
-        n = self.drift_steps
-        bins = "%d bin" % n if n in (1, -1) else "%d bins" % n
-        return "coarse channel = %d, index = %d, snr = %.5f, drift rate = %.5f (%s)" % (
-            self.coarse_channel, self.index, self.snr, self.drift_rate, bins)
-
-    def __repr__(self):
-        return "DedopplerHit(%s)" % self.toString()
-
-    def key(self):
-        return (self.index, self.drift_steps, self.coarse_channel, self.beam)
-
-
 class Dedopplerer:
     """dedoppler.h:15-79, running on the GPU through libseticore_amd.so."""
 
@@ -294,6 +289,9 @@ class Dedopplerer:
         check(L.sc_dd_create(self.num_timesteps, self.num_channels, self.foff, self.tsamp,
                              int(self.has_dc_spike), ctypes.byref(h)))
         self._h = h
+        self._cap = 1024
+        self._hits = (ScHit * self._cap)()
+        self._n = ctypes.c_size_t()
 
     def close(self):
         if getattr(self, "_h", None):
@@ -318,6 +316,9 @@ class Dedopplerer:
         check(lib().sc_dd_set_tuning(self._h, int(channels_per_lane), int(drift_group),
                                      int(prefetch)))
 
+    def set_graphs(self, enable):
+        check(lib().sc_dd_set_graphs(self._h, int(bool(enable))))
+
     def set_timing(self, enable):
         check(lib().sc_dd_set_timing(self._h, int(bool(enable))))
 
@@ -332,19 +333,20 @@ class Dedopplerer:
                snr_threshold, output, stream=None):
         """dedoppler.cu:186-320: appends this coarse channel's hits to `output`."""
         L = lib()
-        meta = metadata._sc_meta()
-        cap = 256
-        hits = (ScHit * cap)()
-        n = ctypes.c_size_t()
-        check(L.sc_dd_search(self._h, ctypes.c_void_p(input.data.data_ptr()), input.dtype,
-                             input.num_timesteps, input.num_channels, input.row_stride,
-                             ctypes.byref(meta), int(beam), int(coarse_channel),
-                             float(max_drift), float(min_drift), float(snr_threshold), hits,
-                             cap, ctypes.byref(n), _stream_ptr(stream)))
-        if n.value > cap:
-            hits = (ScHit * n.value)()
-            check(L.sc_dd_copy_hits(self._h, hits, n.value, ctypes.byref(n)))
-        new = [DedopplerHit._from_c(hits[k]) for k in range(n.value)]
+        n = self._n
+        status = L.sc_dd_search(self._h, input.data.data_ptr(), input.dtype,
+                                input.num_timesteps, input.num_channels, input.row_stride,
+                                ctypes.byref(metadata._sc_meta()), beam, coarse_channel,
+                                max_drift, min_drift, snr_threshold, self._hits, self._cap,
+                                ctypes.byref(n), _stream_ptr(stream))
+        if status != SC_OK:
+            check(status)
+        count = n.value
+        if count > self._cap:
+            self._cap = count
+            self._hits = (ScHit * count)()
+            check(L.sc_dd_copy_hits(self._h, self._hits, count, ctypes.byref(n)))
+        new = list((ScHit * count).from_buffer_copy(self._hits)) if count else []
         if self.print_hits:
             for h in new:
                 print("hit: " + h.toString())
@@ -368,11 +370,11 @@ class Dedopplerer:
 
     def addIncoherentPower(self, input, hits, stream=None):
         """dedoppler.cu:139-176: sorts `hits` by drift steps, fills incoherent_power."""
-        arr = (ScHit * max(1, len(hits)))(*[h._to_c() for h in hits])
+        arr = (ScHit * max(1, len(hits)))(*hits)
         check(lib().sc_dd_add_incoherent_power(
             self._h, ctypes.c_void_p(input.data.data_ptr()), input.dtype, input.num_timesteps,
             input.num_channels, input.row_stride, arr, len(hits), _stream_ptr(stream)))
-        hits[:] = [DedopplerHit._from_c(arr[k]) for k in range(len(hits))]
+        hits[:] = [ScHit.from_buffer_copy(arr[k]) for k in range(len(hits))]
 
     def taylor_tree(self, input, drift_block, stream=None):
         """[T_r][N] path sums of one drift block (optimizedTaylorTree, taylor.cu:333-350)."""
@@ -410,8 +412,7 @@ class _Writer:
                 input = input.detach().float().cpu().numpy()
             keep = np.ascontiguousarray(input, dtype=np.float32)
             ptr = keep.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-        c = hit._to_c()
-        check(lib().sc_writer_record_hit(self._h, ctypes.byref(c), ptr))
+        check(lib().sc_writer_record_hit(self._h, ctypes.byref(hit), ptr))
 
     def close(self):
         if self._h:

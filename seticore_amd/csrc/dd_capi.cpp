@@ -105,6 +105,22 @@ struct sc_dd {
   int4* h_cand = nullptr;
   float* h_stats = nullptr;
 
+  // Graph replay of the device-statistics search (keyed by input and
+  // parameters), captured on a private stream and launched on the caller's.
+  struct GraphEntry {
+    const void* data;
+    int dtype;
+    long stride;
+    int bmin, nblocks, window;
+    double snr;
+    int lanes, group, prefetch;
+    hipGraphExec_t exec;
+  };
+  bool graphs = true;
+  hipStream_t capture_stream = nullptr;
+  std::vector<GraphEntry> graph_cache;
+  static constexpr size_t kMaxGraphs = 256;
+
   // Optional kernel timing (HIP events on the search stream).
   bool timing = false;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -161,6 +177,9 @@ void release(sc_dd* dd) {
   hipHostFree(dd->h_stats);
   if (dd->ev_start) hipEventDestroy(dd->ev_start);
   if (dd->ev_stop) hipEventDestroy(dd->ev_stop);
+  for (auto& g : dd->graph_cache) hipGraphExecDestroy(g.exec);
+  dd->graph_cache.clear();
+  if (dd->capture_stream) hipStreamDestroy(dd->capture_stream);
 }
 
 struct DriftRange {
@@ -411,6 +430,121 @@ sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, in
   return SC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Reset, then the path sums of every drift block into the top keys, plus the
+// column sums (dedoppler.cu:205-227).
+sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_stride,
+                            const DriftRange& r, hipStream_t stream) {
+  const int T = dd->rounded, N = dd->num_channels;
+  const int nblocks = r.bmax - r.bmin + 1;
+  // dedoppler.cu:205 (the reference zeroes the float sums; the key 0 means "no
+  // positive path"), plus the statistics histograms.
+  SC_HIP(sc::launch_reset(dd->top_key, N, dd->stats_hist, stream), "reset");
+  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_start, stream), "timing event");
+  if (T <= 16 && sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch)) {
+    SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
+                                     dd->group, dd->prefetch, dd->top_key, dd->colsum, stream),
+           "fused tree");
+  } else {
+    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
+    for (int b = r.bmin; b <= r.bmax; ++b) {
+      float* sums = nullptr;
+      sc_status s = run_tree(dd, data, dtype, row_stride, b, &sums, stream);
+      if (s != SC_OK) return s;
+      SC_HIP(sc::launch_top_update(sums, T, N, b, r.bmin, dd->top_key, stream), "top update");
+    }
+  }
+  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
+  return SC_OK;
+}
+
+// Device statistics, window merge and the copies of the per-window candidates.
+sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_threshold,
+                              hipStream_t stream) {
+  const int T = dd->rounded, N = dd->num_channels;
+  SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
+                                 dd->stats_out, stream),
+         "device stats");
+  SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
+                                 dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off, dd->cand,
+                                 stream),
+         "window merge");
+  const int nwin = (int)(((long)N + r.window - 1) / r.window);
+  SC_HIP(hipMemcpyAsync(dd->h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost,
+                        stream), "copy candidates");
+  SC_HIP(hipMemcpyAsync(dd->h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
+                        stream), "copy stats");
+  return SC_OK;
+}
+
+// The cached graph for this search, capturing it on first use.
+sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
+                       const DriftRange& r, double snr_threshold, hipGraphExec_t* out) {
+  const int nblocks = r.bmax - r.bmin + 1;
+  for (auto& g : dd->graph_cache) {
+    if (g.data == data && g.dtype == dtype && g.stride == row_stride && g.bmin == r.bmin &&
+        g.nblocks == nblocks && g.window == r.window && g.snr == snr_threshold &&
+        g.lanes == dd->lanes && g.group == dd->group && g.prefetch == dd->prefetch) {
+      *out = g.exec;
+      return SC_OK;
+    }
+  }
+  if (!dd->capture_stream)
+    SC_HIP(hipStreamCreateWithFlags(&dd->capture_stream, hipStreamNonBlocking), "stream");
+  SC_HIP(hipStreamBeginCapture(dd->capture_stream, hipStreamCaptureModeThreadLocal),
+         "begin capture");
+  sc_status s = enqueue_path_sums(dd, data, dtype, row_stride, r, dd->capture_stream);
+  if (s == SC_OK) s = enqueue_device_tail(dd, r, snr_threshold, dd->capture_stream);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(dd->capture_stream, &graph);
+  if (s != SC_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return s;
+  }
+  if (e != hipSuccess) return hip_fail(e, "end capture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(e, "instantiate graph");
+  if (dd->graph_cache.size() >= sc_dd::kMaxGraphs) {
+    hipGraphExecDestroy(dd->graph_cache.front().exec);
+    dd->graph_cache.erase(dd->graph_cache.begin());
+  }
+  dd->graph_cache.push_back({data, dtype, row_stride, r.bmin, nblocks, r.window, snr_threshold,
+                             dd->lanes, dd->group, dd->prefetch, exec});
+  *out = exec;
+  return SC_OK;
+}
+
+// Hits from the per-window candidates (dedoppler.cu:305-317).
+void device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
+                 int coarse_channel, double min_drift) {
+  const int N = dd->num_channels;
+  const float median = dd->h_stats[0], std_dev = dd->h_stats[1];
+  dd->last_median = median;
+  dd->last_std = std_dev;
+  const int nwin = (int)(((long)N + r.window - 1) / r.window);
+  for (int i = 0; i < nwin; ++i) {
+    const int4 c = dd->h_cand[i];
+    if (c.x < 0) continue;
+    float cand_sum;
+    std::memcpy(&cand_sum, &c.y, sizeof(float));
+    int drift_bins = c.z * dd->drift_timesteps + c.w;
+    double drift_rate = drift_bins * dd->resolution;
+    float snr = (cand_sum - median) / std_dev;
+    if (std::fabs(drift_rate) >= min_drift)
+      dd->hits.push_back(make_hit(meta, c.x, drift_bins, drift_rate, snr, beam, coarse_channel,
+                                  dd->num_timesteps, cand_sum));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_timesteps,
                        int num_channels, long row_stride, const sc_meta* meta, int beam,
                        int coarse_channel, double max_drift, double min_drift,
@@ -432,29 +566,13 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
   dd->hits.clear();
   dd->state_valid = false;
 
-  // dedoppler.cu:205 (the reference zeroes the float sums; the key 0 means "no
-  // positive path"), plus the statistics histograms.
-  SC_HIP(sc::launch_reset(dd->top_key, N, dd->stats_hist, stream), "reset");
-  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_start, stream), "timing event");
-  if (T <= 16 && sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch)) {
-    SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
-                                     dd->group, dd->prefetch, dd->top_key, dd->colsum, stream),
-           "fused tree");
-  } else {
-    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
-    for (int b = r.bmin; b <= r.bmax; ++b) {
-      float* sums = nullptr;
-      s = run_tree(dd, data, dtype, row_stride, b, &sums, stream);
-      if (s != SC_OK) return s;
-      SC_HIP(sc::launch_top_update(sums, T, N, b, r.bmin, dd->top_key, stream), "top update");
-    }
-  }
-  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
-
   // Tiny coarse channels take the host statistics: the device selection
   // assumes 0 < ceil(.05N) <= floor(.95N) (the reference's own nth_element
   // calls have empty ranges below a few channels).
-  if (dd->stats_mode == SC_STATS_HOST_EXACT || N < 4096) {
+  const bool host_stats = dd->stats_mode == SC_STATS_HOST_EXACT || N < 4096;
+  if (host_stats) {
+    s = enqueue_path_sums(dd, data, dtype, row_stride, r, stream);
+    if (s != SC_OK) return s;
     SC_HIP(sc::launch_unpack_top(dd->top_key, N, T, r.bmin, dd->top_sum, dd->top_blk,
                                  dd->top_off, stream),
            "unpack top");
@@ -474,34 +592,18 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
     dd->last_std = std_dev;
     host_merge(dd, r, meta, beam, coarse_channel, min_drift, snr_threshold, median, std_dev);
   } else {
-    SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
-                                   dd->stats_out, stream),
-           "device stats");
-    SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
-                                   dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off,
-                                   dd->cand, stream),
-           "window merge");
-    const int nwin = (int)(((long)N + r.window - 1) / r.window);
-    SC_HIP(hipMemcpyAsync(dd->h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost,
-                          stream), "copy candidates");
-    SC_HIP(hipMemcpyAsync(dd->h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
-                          stream), "copy stats");
-    SC_HIP(hipStreamSynchronize(stream), "search");
-    const float median = dd->h_stats[0], std_dev = dd->h_stats[1];
-    dd->last_median = median;
-    dd->last_std = std_dev;
-    for (int i = 0; i < nwin; ++i) {
-      const int4 c = dd->h_cand[i];
-      if (c.x < 0) continue;
-      float cand_sum;
-      std::memcpy(&cand_sum, &c.y, sizeof(float));
-      int drift_bins = c.z * dd->drift_timesteps + c.w;
-      double drift_rate = drift_bins * dd->resolution;
-      float snr = (cand_sum - median) / std_dev;
-      if (std::fabs(drift_rate) >= min_drift)
-        dd->hits.push_back(make_hit(meta, c.x, drift_bins, drift_rate, snr, beam, coarse_channel,
-                                    dd->num_timesteps, cand_sum));
+    if (dd->graphs && !dd->timing) {
+      hipGraphExec_t exec = nullptr;
+      s = search_graph(dd, data, dtype, row_stride, r, snr_threshold, &exec);
+      if (s != SC_OK) return s;
+      SC_HIP(hipGraphLaunch(exec, stream), "graph launch");
+    } else {
+      s = enqueue_path_sums(dd, data, dtype, row_stride, r, stream);
+      if (s == SC_OK) s = enqueue_device_tail(dd, r, snr_threshold, stream);
+      if (s != SC_OK) return s;
     }
+    SC_HIP(hipStreamSynchronize(stream), "search");
+    device_hits(dd, r, meta, beam, coarse_channel, min_drift);
   }
   if (dd->timing) {
     float ms = 0.0f;
@@ -515,6 +617,12 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
     size_t n = std::min(capacity, dd->hits.size());
     std::memcpy(hits, dd->hits.data(), n * sizeof(sc_hit));
   }
+  return SC_OK;
+}
+
+sc_status sc_dd_set_graphs(sc_dd* dd, int enable) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  dd->graphs = enable != 0;
   return SC_OK;
 }
 

@@ -487,134 +487,200 @@ __global__ void unpack_top_kernel(const top_key_t* __restrict__ key, int N, int 
 
 // ---------------------------------------------------------------------------
 // Device statistics (replacing the host's nth_element, dedoppler.cu:241-262):
-// exact order statistics at ranks mid = N/2, first = ceil(.05N) and
-// last = floor(.95N) by a two-level 16-bit radix select, and the trimmed
-// mean/deviation over the sorted ranks [first, last] in FP64.
-//   level 1: histogram of the top 16 bits of each column sum's order key
-//            (built where the column sums are computed);
-//   select:  one workgroup finds the level-1 bin of each rank;
-//   level 2: histograms of the low 16 bits inside those three bins, plus
-//            FP64 sums (relative to a shift K inside the median bin) of the
-//            values strictly between the first and last bins;
-//   final:   one workgroup resolves the exact values, adds the boundary
-//            bins' contributions from their level-2 histograms (every level-2
-//            bin is one exact float), and forms
-//              m   = float(sum) / count          (as the reference forms it)
-//              std = sqrt(sum (x - m)^2 / count) (FP64).
-// All reductions run in a fixed order, so results are deterministic.
+// the exact order statistics at ranks mid = N/2, first = ceil(.05N) and
+// last = floor(.95N), and the trimmed mean/deviation over the sorted ranks
+// [first, last] in FP64, by a three-level radix select over the column sums'
+// order keys (11 + 11 + 10 bits):
+//   pass 1: histogram of the top 11 bits;
+//   pass 2: every workgroup finds the level-1 bin of each rank from that
+//           histogram, then histograms the next 11 bits inside those bins;
+//   pass 3: likewise for the last 10 bits;
+//   pass 4: every workgroup resolves the three exact values, then sums
+//           (x - K) and (x - K)^2 in FP64 over the values strictly between
+//           the first and last ranks' values (K = the median);
+//   final:  one workgroup adds the boundary values' copies that fall inside
+//           [first, last] and forms m = float(sum) / count, as the reference
+//           forms its mean, and std = sqrt(sum (x - m)^2 / count) in FP64.
+// Histograms are counted in LDS and flushed with one atomic per non-zero bin;
+// partial sums are reduced in a fixed order, so results are deterministic.
 
-constexpr int kHistBins = 65536;
 constexpr int kStatsBlocks = 256;
 constexpr int kStatsThreads = 256;
+constexpr int kL1Bits = 11, kL2Bits = 11, kL3Bits = 10;
+constexpr int kL1 = 1 << kL1Bits, kL2 = 1 << kL2Bits, kL3 = 1 << kL3Bits;
+// Histogram words: level 1, then level 2 and level 3 for each of the 3 ranks.
+constexpr int kHistWords = kL1 + 3 * kL2 + 3 * kL3;
 
 struct StatsState {
-  int bin[3];        // level-1 bins of ranks mid, first, last
-  int rank_in[3];    // residual rank inside that bin
-  int below[3];      // number of values in lower level-1 bins
-  float shift;       // K: the median bin's smallest value
-  int pad;
+  unsigned key[3];    // exact order keys of ranks mid, first, last
+  unsigned below[3];  // number of values with a smaller key
+  unsigned equal[3];  // number of values with the same key
   double part_s1[kStatsBlocks], part_s2[kStatsBlocks];
   unsigned part_n[kStatsBlocks];
 };
 
-// Histogram buffers: hist_hi[65536] then hist_lo[3][65536].
-size_t stats_hist_words() { return (size_t)4 * kHistBins; }
+size_t stats_hist_words() { return (size_t)kHistWords; }
 
 __device__ __forceinline__ int rank_first(int N) { return (int)ceil(0.05 * N); }
 __device__ __forceinline__ int rank_last(int N) { return (int)floor(0.95 * N); }
 
-// Finds, for each of three ranks, the bin of a 65536-bin histogram holding it.
-// One workgroup of 1024 threads, 64 bins each.
-__device__ void select_bins(const unsigned* __restrict__ hist, const int (&want)[3],
-                            int (&bin)[3], int (&rank_in)[3], int (&below)[3]) {
-  __shared__ unsigned scan[1024];
-  __shared__ int found[3][3];
+// For each rank r (want[r] counted among the values matching the previous
+// levels), finds the bin of `hist[r]` holding it: bin[r], the number of values
+// in lower bins (added to below[r]) and the bin count. 256 threads; nbins a
+// multiple of 256.
+template <int NBINS>
+__device__ void select_level(const unsigned* __restrict__ hist, int hist_stride,
+                             const unsigned (&want)[3], unsigned (&bin)[3],
+                             unsigned (&below)[3], unsigned (&count)[3]) {
+  constexpr int kPer = NBINS / kStatsThreads;
+  __shared__ unsigned scan[kStatsThreads];
+  __shared__ unsigned found[3][3];
   const int tid = threadIdx.x;
   if (tid < 9) found[tid / 3][tid % 3] = 0;
-  unsigned h[64];
-  unsigned local = 0;
-#pragma unroll
-  for (int i = 0; i < 64; ++i) {
-    h[i] = hist[tid * 64 + i];
-    local += h[i];
-  }
-  scan[tid] = local;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-    const unsigned v = tid >= off ? scan[tid - off] : 0u;
-    __syncthreads();
-    scan[tid] += v;
-    __syncthreads();
-  }
-  const unsigned before = tid ? scan[tid - 1] : 0u;
   for (int r = 0; r < 3; ++r) {
-    const unsigned w = (unsigned)want[r];
+    const unsigned* h = hist + r * hist_stride;
+    unsigned v[kPer];
+    unsigned local = 0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      v[i] = h[tid * kPer + i];
+      local += v[i];
+    }
+    scan[tid] = local;
+    __syncthreads();
+    for (int off = 1; off < kStatsThreads; off <<= 1) {
+      const unsigned x = tid >= off ? scan[tid - off] : 0u;
+      __syncthreads();
+      scan[tid] += x;
+      __syncthreads();
+    }
+    const unsigned before = tid ? scan[tid - 1] : 0u;
+    const unsigned w = want[r];
     if (w >= before && w < scan[tid]) {
       unsigned cum = before;
-      int fi = 0;
-      unsigned fcum = cum;
-      bool done = false;
 #pragma unroll
-      for (int i = 0; i < 64; ++i) {
-        if (!done && w < cum + h[i]) {
-          fi = i;
-          fcum = cum;
-          done = true;
+      for (int i = 0; i < kPer; ++i) {
+        if (w >= cum && w < cum + v[i]) {
+          found[r][0] = (unsigned)(tid * kPer + i);
+          found[r][1] = cum;
+          found[r][2] = v[i];
         }
-        cum += h[i];
+        cum += v[i];
       }
-      found[r][0] = tid * 64 + fi;
-      found[r][1] = (int)(w - fcum);
-      found[r][2] = (int)fcum;
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int r = 0; r < 3; ++r) {
     bin[r] = found[r][0];
-    rank_in[r] = found[r][1];
-    below[r] = found[r][2];
+    below[r] = found[r][1];
+    count[r] = found[r][2];
   }
   __syncthreads();
 }
 
-// Level-1 histogram built apart from the tree kernel: column sums cluster in a
-// few hundred top-16-bit bins, so each workgroup first counts its share in LDS
-// (two 16-bit counters per word, at most 65535 values per workgroup) and then
-// adds its non-zero bins to the global histogram.
-constexpr int kHist1Threads = 1024;
-constexpr int kHist1PerBlock = 32768;
+__device__ __forceinline__ void stats_ranks(int N, unsigned (&want)[3]) {
+  want[0] = (unsigned)(N / 2);
+  want[1] = (unsigned)rank_first(N);
+  want[2] = (unsigned)rank_last(N);
+}
 
-__global__ void __launch_bounds__(kHist1Threads)
-    stats_hist1_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist_hi) {
-  __shared__ unsigned h[kHistBins / 2];
-  for (int i = threadIdx.x; i < kHistBins / 2; i += blockDim.x) h[i] = 0;
+// Counts `sel(key)` into an LDS histogram for every value passing `match`,
+// then flushes the non-zero bins into global memory.
+template <int NBINS, typename Match, typename Digit>
+__device__ void block_histogram(const float* __restrict__ v, int N, unsigned* lds,
+                                unsigned* __restrict__ out, Match match, Digit digit) {
+  for (int i = threadIdx.x; i < 3 * NBINS; i += blockDim.x) lds[i] = 0;
   __syncthreads();
-  const int begin = blockIdx.x * kHist1PerBlock;
-  const int end = min(N, begin + kHist1PerBlock);
-  for (int i = begin + threadIdx.x; i < end; i += blockDim.x) {
-    const unsigned b = ord_key(v[i]) >> 16;
-    atomicAdd(&h[b >> 1], (b & 1u) ? 65536u : 1u);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const unsigned k = ord_key(v[i]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      if (match(r, k)) atomicAdd(&lds[r * NBINS + digit(k)], 1u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kHistBins / 2; i += blockDim.x) {
-    const unsigned w = h[i];
-    if (w & 0xFFFFu) atomicAdd(&hist_hi[2 * i], w & 0xFFFFu);
-    if (w >> 16) atomicAdd(&hist_hi[2 * i + 1], w >> 16);
+  for (int i = threadIdx.x; i < 3 * NBINS; i += blockDim.x)
+    if (lds[i]) atomicAdd(&out[i], lds[i]);
+}
+
+__global__ void __launch_bounds__(kStatsThreads)
+    stats_pass1_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
+  __shared__ unsigned h[kL1];
+  for (int i = threadIdx.x; i < kL1; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+    atomicAdd(&h[ord_key(v[i]) >> (32 - kL1Bits)], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kL1; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Level-1 selection (replicated in every workgroup).
+__device__ void select_l1(const unsigned* __restrict__ hist, int N, unsigned (&pre)[3],
+                          unsigned (&want)[3], unsigned (&below)[3]) {
+  unsigned w0[3], bin[3], bl[3], cnt[3];
+  stats_ranks(N, w0);
+  select_level<kL1>(hist, 0, w0, bin, bl, cnt);
+  for (int r = 0; r < 3; ++r) {
+    pre[r] = bin[r];
+    want[r] = w0[r] - bl[r];
+    below[r] = bl[r];
   }
 }
 
-__global__ void __launch_bounds__(1024)
-    stats_select_kernel(const unsigned* __restrict__ hist_hi, int N, StatsState* st) {
-  const int want[3] = {N / 2, rank_first(N), rank_last(N)};
-  int bin[3], rank_in[3], below[3];
-  select_bins(hist_hi, want, bin, rank_in, below);
-  if (threadIdx.x == 0) {
-    for (int r = 0; r < 3; ++r) {
-      st->bin[r] = bin[r];
-      st->rank_in[r] = rank_in[r];
-      st->below[r] = below[r];
-    }
-    st->shift = ord_val((unsigned)bin[0] << 16);
+__device__ void select_l2(const unsigned* __restrict__ hist, unsigned (&pre)[3],
+                          unsigned (&want)[3], unsigned (&below)[3]) {
+  unsigned bin[3], bl[3], cnt[3];
+  select_level<kL2>(hist + kL1, kL2, want, bin, bl, cnt);
+  for (int r = 0; r < 3; ++r) {
+    pre[r] = (pre[r] << kL2Bits) | bin[r];
+    want[r] -= bl[r];
+    below[r] += bl[r];
+  }
+}
+
+__global__ void __launch_bounds__(kStatsThreads)
+    stats_pass2_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
+  __shared__ unsigned h[3 * kL2];
+  unsigned pre[3], want[3], below[3];
+  select_l1(hist, N, pre, want, below);
+  const unsigned p0 = pre[0], p1 = pre[1], p2 = pre[2];
+  block_histogram<kL2>(
+      v, N, h, hist + kL1,
+      [=](int r, unsigned k) {
+        const unsigned top = k >> (32 - kL1Bits);
+        return top == (r == 0 ? p0 : (r == 1 ? p1 : p2));
+      },
+      [](unsigned k) { return (k >> kL3Bits) & (kL2 - 1); });
+}
+
+__global__ void __launch_bounds__(kStatsThreads)
+    stats_pass3_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
+  __shared__ unsigned h[3 * kL3];
+  unsigned pre[3], want[3], below[3];
+  select_l1(hist, N, pre, want, below);
+  select_l2(hist, pre, want, below);
+  const unsigned p0 = pre[0], p1 = pre[1], p2 = pre[2];
+  block_histogram<kL3>(
+      v, N, h, hist + kL1 + 3 * kL2,
+      [=](int r, unsigned k) {
+        const unsigned top = k >> kL3Bits;
+        return top == (r == 0 ? p0 : (r == 1 ? p1 : p2));
+      },
+      [](unsigned k) { return k & (kL3 - 1); });
+}
+
+// All three levels: exact keys, counts below and equal.
+__device__ void select_all(const unsigned* __restrict__ hist, int N, unsigned (&key)[3],
+                           unsigned (&below)[3], unsigned (&equal)[3]) {
+  unsigned pre[3], want[3];
+  select_l1(hist, N, pre, want, below);
+  select_l2(hist, pre, want, below);
+  unsigned bin[3], bl[3], cnt[3];
+  select_level<kL3>(hist + kL1 + 3 * kL2, kL3, want, bin, bl, cnt);
+  for (int r = 0; r < 3; ++r) {
+    key[r] = (pre[r] << kL3Bits) | bin[r];
+    below[r] += bl[r];
+    equal[r] = cnt[r];
   }
 }
 
@@ -632,23 +698,27 @@ __device__ __forceinline__ V block_sum(V x, V* sh) {
 }
 
 __global__ void __launch_bounds__(kStatsThreads)
-    stats_level2_kernel(const float* __restrict__ v, int N, StatsState* st,
-                        unsigned* __restrict__ hist_lo) {
+    stats_pass4_kernel(const float* __restrict__ v, int N, const unsigned* __restrict__ hist,
+                       StatsState* st) {
   __shared__ double shd[kStatsThreads];
   __shared__ unsigned shu[kStatsThreads];
-  const int b_mid = st->bin[0], b_lo = st->bin[1], b_hi = st->bin[2];
-  const double K = (double)st->shift;
+  unsigned key[3], below[3], equal[3];
+  select_all(hist, N, key, below, equal);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int r = 0; r < 3; ++r) {
+      st->key[r] = key[r];
+      st->below[r] = below[r];
+      st->equal[r] = equal[r];
+    }
+  }
+  const unsigned k_lo = key[1], k_hi = key[2];
+  const double K = (double)ord_val(key[0]);
   double s1 = 0.0, s2 = 0.0;
   unsigned n = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
     const float x = v[i];
     const unsigned k = ord_key(x);
-    const int hi = (int)(k >> 16);
-    const unsigned lo = k & 0xFFFFu;
-    if (hi == b_mid) atomicAdd(&hist_lo[lo], 1u);
-    if (hi == b_lo) atomicAdd(&hist_lo[kHistBins + lo], 1u);
-    if (hi == b_hi) atomicAdd(&hist_lo[2 * kHistBins + lo], 1u);
-    if (hi > b_lo && hi < b_hi) {
+    if (k > k_lo && k < k_hi) {
       const double d = __dsub_rn((double)x, K);
       s1 = __dadd_rn(s1, d);
       s2 = __dadd_rn(s2, __dmul_rn(d, d));
@@ -665,105 +735,46 @@ __global__ void __launch_bounds__(kStatsThreads)
   }
 }
 
-// Adds to (n, s1, s2) the values of level-1 bin `bin` whose level-2 digit lies
-// in [d0, d1) (each digit is one exact float value), relative to K.
-__device__ void add_digit_range(const unsigned* __restrict__ h, int bin, int d0, int d1,
-                                double K, double* sh, unsigned* shu, unsigned& n_out,
-                                double& s1_out, double& s2_out) {
-  double s1 = 0.0, s2 = 0.0;
-  unsigned n = 0;
-  for (int d = d0 + (int)threadIdx.x; d < d1; d += blockDim.x) {
-    const unsigned c = h[d];
-    if (c) {
-      const double x = (double)ord_val(((unsigned)bin << 16) | (unsigned)d);
-      const double dx = __dsub_rn(x, K);
-      s1 = __dadd_rn(s1, __dmul_rn((double)c, dx));
-      s2 = __dadd_rn(s2, __dmul_rn((double)c, __dmul_rn(dx, dx)));
-      n += c;
-    }
-  }
-  s1 = block_sum<double>(s1, sh);
-  s2 = block_sum<double>(s2, sh);
-  n = block_sum<unsigned>(n, shu);
-  n_out += n;
-  s1_out = __dadd_rn(s1_out, s1);
-  s2_out = __dadd_rn(s2_out, s2);
-}
-
-// Exact digit of a residual rank inside a level-2 histogram (all threads).
-__device__ void digit_of_rank(const unsigned* __restrict__ h, int rank_in, int& digit,
-                              int& below_in) {
-  const int want[3] = {rank_in, rank_in, rank_in};
-  int bin[3], r_in[3], below[3];
-  select_bins(h, want, bin, r_in, below);
-  digit = bin[0];
-  below_in = below[0];
-}
-
-__global__ void __launch_bounds__(1024)
-    stats_final_kernel(const StatsState* st, const unsigned* __restrict__ hist_lo, int N,
-                       float* __restrict__ stats_out) {
-  __shared__ double shd[1024];
-  __shared__ unsigned shu[1024];
+__global__ void __launch_bounds__(kStatsThreads)
+    stats_final_kernel(const StatsState* st, int N, float* __restrict__ stats_out) {
+  __shared__ double shd[kStatsThreads];
+  const int tid = threadIdx.x;
+  // Fixed-order tree reduction of the per-workgroup partials.
+  const double s1 = block_sum<double>(st->part_s1[tid], shd);
+  const double s2 = block_sum<double>(st->part_s2[tid], shd);
+  if (tid != 0) return;
   const int first = rank_first(N), last = rank_last(N);
   const long cnt = (long)last + 1 - first;
-  const double K = (double)st->shift;
-  int d_mid, d_lo, d_hi, bl_mid, bl_lo, bl_hi;
-  digit_of_rank(hist_lo, st->rank_in[0], d_mid, bl_mid);
-  digit_of_rank(hist_lo + kHistBins, st->rank_in[1], d_lo, bl_lo);
-  digit_of_rank(hist_lo + 2 * kHistBins, st->rank_in[2], d_hi, bl_hi);
-  const int b_lo = st->bin[1], b_hi = st->bin[2];
-  const float median = ord_val(((unsigned)st->bin[0] << 16) | (unsigned)d_mid);
-  const float v_lo = ord_val(((unsigned)b_lo << 16) | (unsigned)d_lo);
-  const float v_hi = ord_val(((unsigned)b_hi << 16) | (unsigned)d_hi);
-
-  // Values strictly between the first and last level-1 bins.
-  double s1 = 0.0, s2 = 0.0;
-  unsigned n = 0;
-  if (threadIdx.x == 0) {
-    for (int b = 0; b < kStatsBlocks; ++b) {
-      s1 = __dadd_rn(s1, st->part_s1[b]);
-      s2 = __dadd_rn(s2, st->part_s2[b]);
-      n += st->part_n[b];
-    }
-  }
-  // Boundary bins: values strictly inside (v_lo, v_hi) from the level-2
-  // histograms, then the copies of v_lo and v_hi that fall in [first, last].
-  const int below_lo = st->below[1] + bl_lo;  // values < v_lo
-  const int below_hi = st->below[2] + bl_hi;  // values < v_hi
-  const unsigned eq_lo = hist_lo[kHistBins + d_lo];
+  const float median = ord_val(st->key[0]);
+  const float v_lo = ord_val(st->key[1]);
+  const float v_hi = ord_val(st->key[2]);
+  const double K = (double)median;
+  // Copies of the boundary values whose ranks fall in [first, last].
   long k_lo, k_hi;
-  if (b_lo == b_hi) {
-    add_digit_range(hist_lo + kHistBins, b_lo, d_lo + 1, d_hi, K, shd, shu, n, s1, s2);
-  } else {
-    add_digit_range(hist_lo + kHistBins, b_lo, d_lo + 1, kHistBins, K, shd, shu, n, s1, s2);
-    add_digit_range(hist_lo + 2 * kHistBins, b_hi, 0, d_hi, K, shd, shu, n, s1, s2);
-  }
-  if (v_lo == v_hi) {
+  if (st->key[1] == st->key[2]) {
     k_lo = cnt;
     k_hi = 0;
   } else {
-    k_lo = min((long)below_lo + (long)eq_lo, (long)last + 1) - first;
-    k_hi = (long)last + 1 - below_hi;
+    k_lo = min((long)st->below[1] + (long)st->equal[1], (long)last + 1) - first;
+    k_hi = (long)last + 1 - (long)st->below[2];
   }
-  if (threadIdx.x == 0) {
-    const double dl = __dsub_rn((double)v_lo, K), dh = __dsub_rn((double)v_hi, K);
-    s1 = __dadd_rn(s1, __dadd_rn(__dmul_rn((double)k_lo, dl), __dmul_rn((double)k_hi, dh)));
-    s2 = __dadd_rn(s2, __dadd_rn(__dmul_rn((double)k_lo, __dmul_rn(dl, dl)),
-                                 __dmul_rn((double)k_hi, __dmul_rn(dh, dh))));
-    // float sum = accumulate(...) in double; float m = sum / count.
-    const double total = __dadd_rn(__dmul_rn((double)cnt, K), s1);
-    const float sum_f = (float)total;
-    const float m = sum_f / (float)cnt;
-    const double delta = __dsub_rn((double)m, K);
-    // sum (x - m)^2 = sum (x-K)^2 - 2 delta sum (x-K) + cnt delta^2
-    const double acc = __dadd_rn(__dsub_rn(s2, __dmul_rn(2.0 * delta, s1)),
-                                 __dmul_rn((double)cnt, __dmul_rn(delta, delta)));
-    stats_out[0] = median;
-    stats_out[1] = (float)sqrt(fmax(acc, 0.0) / (double)cnt);
-    stats_out[2] = m;
-    stats_out[3] = (float)n;
-  }
+  const double dl = __dsub_rn((double)v_lo, K), dh = __dsub_rn((double)v_hi, K);
+  const double t1 =
+      __dadd_rn(s1, __dadd_rn(__dmul_rn((double)k_lo, dl), __dmul_rn((double)k_hi, dh)));
+  const double t2 = __dadd_rn(s2, __dadd_rn(__dmul_rn((double)k_lo, __dmul_rn(dl, dl)),
+                                            __dmul_rn((double)k_hi, __dmul_rn(dh, dh))));
+  // float sum = accumulate(...) (in double); float m = sum / count.
+  const double total = __dadd_rn(__dmul_rn((double)cnt, K), t1);
+  const float sum_f = (float)total;
+  const float m = sum_f / (float)cnt;
+  const double delta = __dsub_rn((double)m, K);
+  // sum (x - m)^2 = sum (x-K)^2 - 2 delta sum (x-K) + cnt delta^2
+  const double acc = __dadd_rn(__dsub_rn(t2, __dmul_rn(2.0 * delta, t1)),
+                               __dmul_rn((double)cnt, __dmul_rn(delta, delta)));
+  stats_out[0] = median;
+  stats_out[1] = (float)sqrt(fmax(acc, 0.0) / (double)cnt);
+  stats_out[2] = m;
+  stats_out[3] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1153,13 +1164,12 @@ size_t device_stats_hist_bytes() { return sizeof(unsigned) * stats_hist_words();
 hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsigned* hist,
                                float* stats_out, hipStream_t stream) {
   StatsState* st = (StatsState*)scratch;
-  hipLaunchKernelGGL(stats_hist1_kernel, dim3((N + kHist1PerBlock - 1) / kHist1PerBlock),
-                     dim3(kHist1Threads), 0, stream, colsum, N, hist);
-  hipLaunchKernelGGL(stats_select_kernel, dim3(1), dim3(1024), 0, stream, hist, N, st);
-  hipLaunchKernelGGL(stats_level2_kernel, dim3(kStatsBlocks), dim3(kStatsThreads), 0, stream,
-                     colsum, N, st, hist + kHistBins);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(1024), 0, stream, st, hist + kHistBins, N,
-                     stats_out);
+  const dim3 grid(kStatsBlocks), block(kStatsThreads);
+  hipLaunchKernelGGL(stats_pass1_kernel, grid, block, 0, stream, colsum, N, hist);
+  hipLaunchKernelGGL(stats_pass2_kernel, grid, block, 0, stream, colsum, N, hist);
+  hipLaunchKernelGGL(stats_pass3_kernel, grid, block, 0, stream, colsum, N, hist);
+  hipLaunchKernelGGL(stats_pass4_kernel, grid, block, 0, stream, colsum, N, hist, st);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), block, 0, stream, st, N, stats_out);
   return hipGetLastError();
 }
 

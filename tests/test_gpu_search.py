@@ -263,3 +263,35 @@ def test_shape_mismatch_is_an_error(gpu):
     with pytest.raises(sc.SeticoreError):
         dd.search(sc.FilterbankBuffer(8, 1000), sc.FilterbankMetadata(), -1, 0, 1.0, 0.0, 10.0,
                   [])
+
+
+def test_graph_replay_matches_eager(gpu, oracle):
+    """Device-statistics searches replay a captured graph; every replay (with
+    eager work in between) must equal the eager launch sequence."""
+    torch = _torch()
+    T, N = 16, 1 << 17
+    datas = [_waterfall(T, N, 40 + k, [(5000.0 + 999 * k, 1.7, 4.0)]) for k in range(2)]
+    bufs = [sc.FilterbankBuffer(T, N, data=_to_gpu(d)) for d in datas]
+    meta = sc.FilterbankMetadata(coarse_channel_size=N)
+    eager = sc.Dedopplerer(T, N, -2.79e-6, 18.25)
+    eager.set_stats_mode(sc.SC_STATS_DEVICE)
+    eager.set_graphs(False)
+    want = []
+    for k in range(2):
+        hits = []
+        eager.search(bufs[k], meta, -1, k, 3.0, 0.0, 6.0, hits)
+        want.append(([h.key() + (h.snr, h.power) for h in hits], eager.state()))
+    dd = sc.Dedopplerer(T, N, -2.79e-6, 18.25)
+    dd.set_stats_mode(sc.SC_STATS_DEVICE)
+    for rep in range(4):
+        for k in range(2):
+            hits = []
+            dd.search(bufs[k], meta, -1, k, 3.0, 0.0, 6.0, hits)
+            got = [h.key() + (h.snr, h.power) for h in hits]
+            assert got == want[k][0], (rep, k)
+            st = dd.state()
+            np.testing.assert_array_equal(st["top_sums"], want[k][1]["top_sums"])
+            np.testing.assert_array_equal(st["column_sums"], want[k][1]["column_sums"])
+            assert st["median"] == want[k][1]["median"] and st["std"] == want[k][1]["std"]
+            torch.cuda.synchronize()
+            _ = (bufs[k].data * 2.0).sum().item()  # eager work between replays
