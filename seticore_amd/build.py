@@ -41,6 +41,25 @@ def _compile(src, extra):
     return obj
 
 
+ROOT = os.path.dirname(HERE)
+KAT_SRC = os.path.join(ROOT, "tests", "cpp", "dedoppler_kat.cpp")
+KAT_BIN = os.path.join(ROOT, "tests", "cpp", "dedoppler_kat")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def build_cpp_tests():
+    """The C++ façade's known-answer test program (links the library)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__", KAT_SRC,
+           "-L", HERE, "-lseticore_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../seticore_amd", "-Wl,-rpath," + os.path.join(ROCM, "lib"),
+           "-o", KAT_BIN]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("C++ test build failed: %s\n%s" % (" ".join(cmd), res.stderr))
+    return KAT_BIN
+
+
 def build(verbose=False):
     """Compiles every source and links the shared library. Always rebuilds."""
     os.makedirs(BUILD_DIR, exist_ok=True)
@@ -52,6 +71,7 @@ def build(verbose=False):
     if res.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), res.stdout, res.stderr))
     os.replace(tmp, LIB)
+    build_cpp_tests()
     if verbose:
         print("built", LIB)
     return LIB

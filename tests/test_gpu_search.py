@@ -295,3 +295,21 @@ def test_graph_replay_matches_eager(gpu, oracle):
             assert st["median"] == want[k][1]["median"] and st["std"] == want[k][1]["std"]
             torch.cuda.synchronize()
             _ = (bufs[k].data * 2.0).sum().item()  # eager work between replays
+
+
+def test_cpp_facade_known_answer(gpu, tmp_path):
+    """dedoppler_test.cpp:9-33 through the C++ façade (include/seticore_amd.hpp),
+    then both writers on its hit."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    from seticore_amd.hitfile import read_hits
+    exe = os.path.join(ROOT, "tests", "cpp", "dedoppler_kat")
+    assert os.path.exists(exe), "build the C++ test with seticore_amd.build"
+    res = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert res.stdout.startswith("ok coarse channel = 555, index = 70")
+    dat = open(tmp_path / "kat.dat").read().splitlines()
+    assert dat[-1].split("\t")[5] == "70"
+    hits = read_hits(str(tmp_path / "kat.hits"))
+    assert len(hits) == 1 and hits[0]["signal"]["drift_steps"] == 3
