@@ -97,6 +97,15 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
  * used by the autotuner. */
 sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, int prefetch);
 
+/* Shape of the two-pass search used when rounded_timesteps is 32..256 (the
+ * reference's tiledTaylorTree<32> / twoStageTaylorTree range, taylor.cu:261-350):
+ * drift blocks per LDS window in the level-16 pass (4 or 8) and drift blocks
+ * per launch pair (1..64; 0 = as many as keep the scratch within 256 MiB, at
+ * most drift_group). May grow the scratch (and drops cached graphs when it
+ * does). Results do not depend on the shape. No counterpart in the reference;
+ * used by the autotuner. */
+sc_status sc_dd_set_two_pass(sc_dd* dd, int drift_group, int blocks_per_launch);
+
 /* Whether device-statistics searches replay a cached hipGraph of their
  * launch sequence (one graph per input pointer and search parameters, up to
  * 256; captured on first use). Default on. No counterpart in the reference. */

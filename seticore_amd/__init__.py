@@ -114,6 +114,7 @@ def lib():
         "sc_dd_memory_usage": [vp, P(sz)],
         "sc_dd_set_stats_mode": [vp, i],
         "sc_dd_set_tuning": [vp, i, i, i],
+        "sc_dd_set_two_pass": [vp, i, i],
         "sc_dd_set_timing": [vp, i],
         "sc_dd_set_graphs": [vp, i],
         "sc_dd_kernel_time": [vp, P(d), P(lng)],
@@ -315,6 +316,10 @@ class Dedopplerer:
     def set_tuning(self, channels_per_lane, drift_group, prefetch=0):
         check(lib().sc_dd_set_tuning(self._h, int(channels_per_lane), int(drift_group),
                                      int(prefetch)))
+
+    def set_two_pass(self, drift_group, blocks_per_launch=0):
+        """Shape of the T_r = 32..256 two-pass search (speed only)."""
+        check(lib().sc_dd_set_two_pass(self._h, int(drift_group), int(blocks_per_launch)))
 
     def set_graphs(self, enable):
         check(lib().sc_dd_set_graphs(self._h, int(bool(enable))))

@@ -92,6 +92,13 @@ struct sc_dd {
   size_t cand_capacity = 0;
   float* tree_a = nullptr;     // T_r*N scratch for the staged path and tree API
   float* tree_b = nullptr;
+  // Two-pass search (T_r = 32..256): scratch of pass_blocks * T_r * pass_pitch
+  // floats, pass_group drift blocks per LDS window in the first pass.
+  float* pass_scratch = nullptr;
+  long pass_pitch = 0;
+  int pass_group = 4;
+  int pass_blocks = 0;  // drift blocks per launch pair; 0 = from the size
+  int pass_alloc_blocks = 0;
   int* gather_idx = nullptr;   // 2*capacity ints
   float* gather_out = nullptr;
   size_t gather_capacity = 0;
@@ -114,6 +121,7 @@ struct sc_dd {
     int bmin, nblocks, window;
     double snr;
     int lanes, group, prefetch;
+    int pass_group, pass_blocks;
     hipGraphExec_t exec;
   };
   bool graphs = true;
@@ -167,6 +175,7 @@ void release(sc_dd* dd) {
   hipFree(dd->cand);
   hipFree(dd->tree_a);
   hipFree(dd->tree_b);
+  hipFree(dd->pass_scratch);
   hipFree(dd->gather_idx);
   hipFree(dd->gather_out);
   hipHostFree(dd->h_colsum);
@@ -204,6 +213,30 @@ sc_status ensure_tree_scratch(sc_dd* dd) {
   sc_status s = dev_alloc(dd, (void**)&dd->tree_a, bytes, "tree scratch");
   if (s != SC_OK) return s;
   return dev_alloc(dd, (void**)&dd->tree_b, bytes, "tree scratch");
+}
+
+// Drift blocks per launch pair of the two-pass search: as set, or as many as
+// keep the scratch within 256 MiB (at least one, at most the pass-1 group).
+int pass_blocks(const sc_dd* dd) {
+  if (dd->pass_blocks > 0) return dd->pass_blocks;
+  const double per_block = 4.0 * dd->rounded * (double)dd->pass_pitch;
+  int nb = (int)((256.0 * (1 << 20)) / per_block);
+  return std::max(1, std::min(dd->pass_group, nb));
+}
+
+sc_status ensure_pass_scratch(sc_dd* dd, int nb) {
+  if (dd->pass_scratch && dd->pass_alloc_blocks >= nb) return SC_OK;
+  if (dd->pass_scratch) {
+    hipFree(dd->pass_scratch);
+    dd->device_bytes -= sizeof(float) * (size_t)dd->pass_alloc_blocks * dd->rounded *
+                        (size_t)dd->pass_pitch;
+    dd->pass_scratch = nullptr;
+    dd->pass_alloc_blocks = 0;
+  }
+  const size_t bytes = sizeof(float) * (size_t)nb * dd->rounded * (size_t)dd->pass_pitch;
+  sc_status s = dev_alloc(dd, (void**)&dd->pass_scratch, bytes, "two-pass scratch");
+  if (s == SC_OK) dd->pass_alloc_blocks = nb;
+  return s;
 }
 
 // Path sums of one drift block into a [T_r][N] buffer; returns which buffer.
@@ -380,7 +413,12 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
   if (s == SC_OK)
     s = host_alloc((void**)&dd->h_cand, dd->cand_capacity * sizeof(int4), "candidates");
   if (s == SC_OK) s = host_alloc((void**)&dd->h_stats, 4 * sizeof(float), "stats");
-  if (s == SC_OK && rounded > 16) s = ensure_tree_scratch(dd);
+  if (sc::two_pass_supported(rounded, dd->pass_group)) {
+    dd->pass_pitch = sc::two_pass_row_pitch(rounded, num_channels);
+    if (s == SC_OK) s = ensure_pass_scratch(dd, pass_blocks(dd));
+  } else if (s == SC_OK && rounded > 16) {
+    s = ensure_tree_scratch(dd);
+  }
   if (s == SC_OK) {
     // Block/offset arrays persist across searches like the reference's
     // (dedoppler.cu:205 only resets the sums); start them at zero.
@@ -430,6 +468,24 @@ sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, in
   return SC_OK;
 }
 
+sc_status sc_dd_set_two_pass(sc_dd* dd, int drift_group, int blocks_per_launch) {
+  if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
+  if (!sc::two_pass_supported(dd->rounded, drift_group))
+    return fail(SC_ERR_UNSUPPORTED, "no two-pass kernel for this (T_r, group)");
+  if (blocks_per_launch < 0 || blocks_per_launch > 64)
+    return fail(SC_ERR_INVALID_ARGUMENT, "blocks_per_launch must be in [0, 64]");
+  dd->pass_group = drift_group;
+  dd->pass_blocks = blocks_per_launch;
+  const int nb = pass_blocks(dd);
+  if (nb > dd->pass_alloc_blocks) {
+    // Cached graphs hold the old scratch address.
+    for (auto& g : dd->graph_cache) hipGraphExecDestroy(g.exec);
+    dd->graph_cache.clear();
+    return ensure_pass_scratch(dd, nb);
+  }
+  return SC_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -448,6 +504,17 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
     SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
                                      dd->group, dd->prefetch, dd->top_key, dd->colsum, stream),
            "fused tree");
+  } else if (sc::two_pass_supported(T, dd->pass_group)) {
+    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
+    const int nb = pass_blocks(dd);
+    sc_status s = ensure_pass_scratch(dd, nb);
+    if (s != SC_OK) return s;
+    for (int b = r.bmin; b <= r.bmax; b += nb) {
+      SC_HIP(sc::launch_two_pass(data, dtype, row_stride, T, N, r.bmin, b,
+                                 std::min(nb, r.bmax - b + 1), dd->pass_group,
+                                 dd->pass_scratch, dd->pass_pitch, dd->top_key, stream),
+             "two-pass tree");
+    }
   } else {
     SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
     for (int b = r.bmin; b <= r.bmax; ++b) {
@@ -487,7 +554,8 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
   for (auto& g : dd->graph_cache) {
     if (g.data == data && g.dtype == dtype && g.stride == row_stride && g.bmin == r.bmin &&
         g.nblocks == nblocks && g.window == r.window && g.snr == snr_threshold &&
-        g.lanes == dd->lanes && g.group == dd->group && g.prefetch == dd->prefetch) {
+        g.lanes == dd->lanes && g.group == dd->group && g.prefetch == dd->prefetch &&
+        g.pass_group == dd->pass_group && g.pass_blocks == pass_blocks(dd)) {
       *out = g.exec;
       return SC_OK;
     }
@@ -514,7 +582,8 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
     dd->graph_cache.erase(dd->graph_cache.begin());
   }
   dd->graph_cache.push_back({data, dtype, row_stride, r.bmin, nblocks, r.window, snr_threshold,
-                             dd->lanes, dd->group, dd->prefetch, exec});
+                             dd->lanes, dd->group, dd->prefetch, dd->pass_group,
+                             pass_blocks(dd), exec});
   *out = exec;
   return SC_OK;
 }

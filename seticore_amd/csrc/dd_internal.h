@@ -27,6 +27,16 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
 // Whether launch_fused_tree_top has a kernel for (T, C, G, P).
 bool fused_supported(int T, int C, int G, int P);
 
+// T_r = 32..256: path sums of drift blocks [b_first, b_first + nb) folded into
+// the top keys by two passes through scratch (nb * T * row_pitch floats,
+// row_pitch from two_pass_row_pitch). G = drift blocks per LDS window in the
+// first pass (4 or 8). Column sums are not written.
+bool two_pass_supported(int T, int G);
+long two_pass_row_pitch(int T, int N);
+hipError_t launch_two_pass(const void* x, int dtype, long stride, int T, int N, int bmin,
+                           int b_first, int nb, int G, float* scratch, long row_pitch,
+                           top_key_t* top_key, hipStream_t stream);
+
 // Level-min(T,16) sums for every time block of one drift block:
 // out[(tb*Tb + p)*N + c], Tb = min(T, 16), tb < T/Tb.
 hipError_t launch_tree_blocks(const void* x, int dtype, long stride, int T, int N,

@@ -406,6 +406,179 @@ __global__ void __launch_bounds__(kWave)
   }
 }
 
+// ---------------------------------------------------------------------------
+// T_r = 16*M (M = 2..16): two passes through a scratch array.
+//
+// Unrolling the recursion (taylor.h:53-54,80-82) from level T down to level
+// 16, both halves of every stage read the same offset p/2, so output offset
+// p = q*M + r (q < 16, r < M) reads offset q of the level-16 tree of each of
+// the M 16-row time blocks, and the shift of level 16*2^j,
+// (p_j + 1)/2 with p_j = q*2^j + (r >> (log2 M - j)), is q*2^(j-1) plus the
+// shift of an M-row tree at offset r >> (log2 M - j). So with
+//   R_q[k][c] = level-16 tree of time block k, offset q, start channel
+//               c + k*(16*b + q)      (time block k's own coordinates,
+//               sheared by 16*k*b as the reference's one-step kernel does with
+//               drift*L/2, taylor.cu:97-109, plus k*q),
+// path (b, q*M + r, c) is offset r at channel c of the M-row tree over
+// R_q[0..M-1] at drift 0, added in the same pairwise order as the reference.
+// Pass 1 (level16_kernel) writes R_q for a group of drift blocks; pass 2
+// (upper_tree_kernel) runs the M-row trees from registers, masks invalid
+// paths (dedoppler.cu:40-46) and folds the keys.
+//
+// Scratch layout: row ((gi*16 + q)*M + k) of row_pitch floats holds R_q[k] of
+// drift block b_first + gi. Entries that no valid path reads are not written.
+
+template <int C, int G, int gp>
+__device__ __forceinline__ void level16_block(lds_float* const (&rp)[16], int g, int ng, int b0,
+                                              int gi0, int k, int c_w, int N, int M,
+                                              long row_pitch, float* __restrict__ scratch) {
+  if constexpr (gp < C) {
+    if (g + gp < ng) {
+      const int b = b0 + g + gp;
+      float v[16][C];
+      read_block<16, C, G, gp>(rp, v);
+      tree_regs<16, C>(v);
+      const int lane = threadIdx.x;
+      const int c = c_w + lane * C;
+      if (lane < TileShape<16, C>::kValidLanes && c < N) {
+        float* base = scratch + (long)((gi0 + g + gp) * 16 * M + k) * row_pitch;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const long col = (long)c - (long)k * (16 * b + q);
+          float* dst = base + (long)q * M * row_pitch;
+          if (col >= 0 && col + C <= row_pitch) {
+            if constexpr (C == 4) {
+              typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+              f4u w = {v[q][0], v[q][1], v[q][2], v[q][3]};
+              *(f4u*)(dst + col) = w;
+            } else {
+#pragma unroll
+              for (int j = 0; j < C; ++j) dst[col + j] = v[q][j];
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < C; ++j)
+              if (col + j >= 0 && col + j < row_pitch) dst[col + j] = v[q][j];
+          }
+        }
+      }
+    }
+    level16_block<C, G, gp + 1>(rp, g, ng, b0, gi0, k, c_w, N, M, row_pitch, scratch);
+  }
+}
+
+// Pass 1. One 64-lane workgroup per (channel tile, time block k, group of G
+// drift blocks): the 16-row window of time block k is staged in LDS once for
+// the group, as in fused_tree_top_kernel.
+template <int C, int G, typename In>
+__global__ void __launch_bounds__(kWave)
+    level16_kernel(const In* __restrict__ x, long stride, int N, int M, int b_first, int nb,
+                   int ntiles, int total_items, long row_pitch, float* __restrict__ scratch) {
+  using Sh = TileShape<16, C>;
+  using Wn = Window<16, C, G>;
+  __shared__ float win_storage[Wn::kFloats];
+  lds_float* win = (lds_float*)win_storage;
+  const int per_xcd = gridDim.x / 8;
+  const int item = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (item >= total_items) return;
+  const int tile = item % ntiles;
+  const int rest = item / ntiles;
+  const int k = rest % M;
+  const int grp = rest / M;
+  const int c_w = tile * Sh::kOut;
+  const int gi0 = grp * G;
+  const int b0 = b_first + gi0;
+  const int ng = min(G, nb - gi0);
+  load_window<16, C, G, In>(win, x, stride, N, c_w, b0, 16 * k);
+  __syncthreads();
+  lds_float* rp[16];
+  row_bases<16, C, G>(win, rp);
+  for (int g = 0; g < ng; g += C) {
+    level16_block<C, G, 0>(rp, g, ng, b0, gi0, k, c_w, N, M, row_pitch, scratch);
+#pragma unroll
+    for (int t = 1; t < 16; ++t) rp[t] += t;
+  }
+}
+
+constexpr int kUpperWaves = 4;
+
+// Pass 2. One workgroup of kUpperWaves waves per channel tile; the waves share
+// the (drift block, q) pairs of the group, keep their own key maxima, and
+// merge them through LDS before one atomic max per channel.
+template <int M, int C>
+__global__ void __launch_bounds__(kWave * kUpperWaves)
+    upper_tree_kernel(const float* __restrict__ scratch, long row_pitch, int T, int N, int bmin,
+                      int b_first, int nb, int total_items, top_key_t* __restrict__ top_key) {
+  using Sh = TileShape<M, C>;
+  __shared__ double red[kUpperWaves - 1][kWave * C];
+  const int per_xcd = gridDim.x / 8;
+  const int item = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (item >= total_items) return;
+  const int c_w = item * Sh::kOut;
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  double best[C];
+#pragma unroll
+  for (int j = 0; j < C; ++j) best[j] = __longlong_as_double((long long)kEmptyKey);
+  const bool tile_in = c_w + Sh::kOut - 1 < N;
+  for (int pair = wave; pair < nb * 16; pair += kUpperWaves) {
+    const int gi = pair / 16;
+    const int q = pair % 16;
+    const int b = b_first + gi;
+    const float* src = scratch + (long)pair * M * row_pitch + c_w + lane * C;
+    float v[M][C];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      if constexpr (C == 4) {
+        const float4 w = *(const float4*)(src + (long)k * row_pitch);
+        v[k][0] = w.x;
+        v[k][1] = w.y;
+        v[k][2] = w.z;
+        v[k][3] = w.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < C; ++j) v[k][j] = src[(long)k * row_pitch + j];
+      }
+    }
+    tree_regs<M, C>(v);
+    const bool interior = tile_in && (c_w + (T - 1) * b >= 0) &&
+                          (c_w + Sh::kOut - 1 + (T - 1) * b + (T - 1) < N);
+    if (!interior) {
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        const int c = c_w + lane * C + j;
+        const int last0 = c + (T - 1) * b;
+        const int pmax = (c >= N || last0 < 0) ? 0 : min(T, N - last0);
+#pragma unroll
+        for (int r = 0; r < M; ++r)
+          if (q * M + r >= pmax) v[r][j] = 0.0f;
+      }
+    }
+    const unsigned low0 = (unsigned)kCodeMask - (unsigned)((b - bmin) * T + q * M);
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const unsigned low = low0 - (unsigned)r;
+#pragma unroll
+      for (int j = 0; j < C; ++j) best[j] = fmax(best[j], make_key(v[r][j], low));
+    }
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) red[wave - 1][lane * C + j] = best[j];
+  }
+  __syncthreads();
+  if (wave == 0 && lane < Sh::kValidLanes) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+#pragma unroll
+      for (int w = 0; w < kUpperWaves - 1; ++w) best[j] = fmax(best[j], red[w][lane * C + j]);
+      const int c = c_w + lane * C + j;
+      const unsigned long long key = (unsigned long long)__double_as_longlong(best[j]);
+      if (c < N && key > kEmptyKey) atomicMax(&top_key[c], key);
+    }
+  }
+}
+
 // taylor.cu:97-109 + taylor.h:30-86: one stage in global memory.
 __global__ void one_step_kernel(const float* __restrict__ src, float* __restrict__ dst, int T,
                                 int N, int L, int drift) {
@@ -1046,6 +1219,58 @@ hipError_t fused_dispatch(const In* x, long stride, int T, int N, int bmin, int 
   return hipErrorInvalidValue;
 }
 
+constexpr int kPass1C = 4;
+constexpr int kPass2C = 4;
+
+template <int M>
+long two_pass_pitch_typed(int N) {
+  using Sh = TileShape<M, kPass2C>;
+  const long ntiles = ((long)N + Sh::kOut - 1) / Sh::kOut;
+  return (ntiles * Sh::kOut + kWave * kPass2C + 63) / 64 * 64;
+}
+
+template <int M, int G, typename In>
+hipError_t two_pass_typed(const In* x, long stride, int N, int bmin, int b_first, int nb,
+                          float* scratch, long row_pitch, top_key_t* top_key,
+                          hipStream_t stream) {
+  constexpr int T = 16 * M;
+  if (row_pitch < two_pass_pitch_typed<M>(N)) return hipErrorInvalidValue;
+  using Sh1 = TileShape<16, kPass1C>;
+  const long ntiles1 = ((long)N + Sh1::kOut - 1) / Sh1::kOut;
+  const long ngroups = ((long)nb + G - 1) / G;
+  const long total1 = ntiles1 * M * ngroups;
+  using Sh2 = TileShape<M, kPass2C>;
+  const long total2 = ((long)N + Sh2::kOut - 1) / Sh2::kOut;
+  if (total1 > (1L << 30) || total2 > (1L << 30)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((level16_kernel<kPass1C, G, In>), dim3((int)((total1 + 7) / 8 * 8)),
+                     dim3(kWave), 0, stream, x, stride, N, M, b_first, nb, (int)ntiles1,
+                     (int)total1, row_pitch, scratch);
+  hipLaunchKernelGGL((upper_tree_kernel<M, kPass2C>), dim3((int)((total2 + 7) / 8 * 8)),
+                     dim3(kWave * kUpperWaves), 0, stream, scratch, row_pitch, T, N, bmin,
+                     b_first, nb, (int)total2, top_key);
+  return hipGetLastError();
+}
+
+template <typename In>
+hipError_t two_pass_dispatch(const In* x, long stride, int T, int N, int bmin, int b_first,
+                             int nb, int G, float* scratch, long row_pitch, top_key_t* top_key,
+                             hipStream_t stream) {
+#define SC_TWO_PASS_CASE(MM, GG)                                                          \
+  if (T == 16 * MM && G == GG)                                                            \
+    return two_pass_typed<MM, GG, In>(x, stride, N, bmin, b_first, nb, scratch, row_pitch, \
+                                      top_key, stream);
+  SC_TWO_PASS_CASE(2, 4)
+  SC_TWO_PASS_CASE(2, 8)
+  SC_TWO_PASS_CASE(4, 4)
+  SC_TWO_PASS_CASE(4, 8)
+  SC_TWO_PASS_CASE(8, 4)
+  SC_TWO_PASS_CASE(8, 8)
+  SC_TWO_PASS_CASE(16, 4)
+  SC_TWO_PASS_CASE(16, 8)
+#undef SC_TWO_PASS_CASE
+  return hipErrorInvalidValue;
+}
+
 template <int T, typename In>
 hipError_t tree_blocks_typed(const In* x, long stride, int Tall, int N, int drift, float* out,
                              hipStream_t stream) {
@@ -1094,6 +1319,39 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
     case 2:
       return fused_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N, bmin,
                                             nblocks, C, G, P, top_key, colsum, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+bool two_pass_supported(int T, int G) {
+  return (T == 32 || T == 64 || T == 128 || T == 256) && (G == 4 || G == 8);
+}
+
+long two_pass_row_pitch(int T, int N) {
+  switch (T) {
+    case 32: return two_pass_pitch_typed<2>(N);
+    case 64: return two_pass_pitch_typed<4>(N);
+    case 128: return two_pass_pitch_typed<8>(N);
+    case 256: return two_pass_pitch_typed<16>(N);
+  }
+  return 0;
+}
+
+hipError_t launch_two_pass(const void* x, int dtype, long stride, int T, int N, int bmin,
+                           int b_first, int nb, int G, float* scratch, long row_pitch,
+                           top_key_t* top_key, hipStream_t stream) {
+  if (nb <= 0) return hipSuccess;
+  switch (dtype) {
+    case 0:
+      return two_pass_dispatch<float>((const float*)x, stride, T, N, bmin, b_first, nb, G,
+                                      scratch, row_pitch, top_key, stream);
+    case 1:
+      return two_pass_dispatch<__half>((const __half*)x, stride, T, N, bmin, b_first, nb, G,
+                                       scratch, row_pitch, top_key, stream);
+    case 2:
+      return two_pass_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N, bmin,
+                                               b_first, nb, G, scratch, row_pitch, top_key,
+                                               stream);
   }
   return hipErrorInvalidValue;
 }

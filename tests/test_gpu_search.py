@@ -122,6 +122,9 @@ CASES = [
     (2, 5000, 1.0, 1.0, 2e6, 4.0, 6),
     (32, 20000, -2.79e-6, 18.25, 1.0, 6.0, 7),
     (64, 8192, -2.79e-6, 18.25, 0.5, 6.0, 8),
+    (128, 12000, 2.79e-6, 18.25, 0.6, 6.0, 10),
+    (200, 16384, -2.79e-6, 18.25, 0.5, 6.0, 11),  # T_r = 256 with 56 zero rows
+    (256, 3000, -2.79e-6, 18.25, 2.0, 5.0, 12),   # drift range wider than the channel
     (16, 1000, -2.79e-6, 18.25, 10.0, 4.0, 9),  # drift range wider than the channel
 ]
 
@@ -152,6 +155,33 @@ def test_tunings_are_bit_identical(gpu, oracle, tuning):
     data = _waterfall(T, N, 21, [(1000.5, 1.3, 3.0), (30000.0, -2.2, 2.0)])
     hits, st, ref = _compare_search(oracle, data, 16, -2.79e-6, 18.25, 3.0, 0.0, 6.0,
                                     tuning=tuning)
+    np.testing.assert_array_equal(st["top_sums"], ref.top_sums)
+    pos = ref.top_sums > 0
+    np.testing.assert_array_equal(st["top_blocks"][pos], ref.top_blocks[pos])
+    np.testing.assert_array_equal(st["top_offsets"][pos], ref.top_offsets[pos])
+    assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
+
+
+@pytest.mark.parametrize("T", [32, 64, 128, 256])
+@pytest.mark.parametrize("shape", [(4, 1), (4, 3), (8, 0), (8, 8)])
+def test_two_pass_shapes_are_bit_identical(gpu, oracle, T, shape):
+    # Blocks per launch 1, 3 (a ragged last launch pair), auto and 8, and both
+    # level-16 window groups, against the oracle.
+    N = 6000 if T < 256 else 3000
+    print("two-pass T=%d shape=%s" % (T, shape), flush=True)
+    rng = np.random.default_rng(T)
+    tones = [(rng.uniform(0, N), rng.uniform(-2, 2), rng.uniform(1, 3)) for _ in range(6)]
+    data = _waterfall(T, N, 40 + T, tones)
+    dd = sc.Dedopplerer(T, N, -2.79e-6, 18.25)
+    dd.set_two_pass(*shape)
+    buf = sc.FilterbankBuffer(T, N, data=_to_gpu(data))
+    meta = sc.FilterbankMetadata(fch1=8000.0, foff=-2.79e-6, coarse_channel_size=N)
+    max_drift = 0.25 * 8  # 8+ drift blocks at every T
+    hits = []
+    dd.search(buf, meta, -1, 0, max_drift, 0.0, 5.0, hits)
+    st = dd.state()
+    ref = oracle.search(data, T, -2.79e-6, 18.25, max_drift, 0.0, 5.0, fch1=8000.0,
+                        num_threads=8)
     np.testing.assert_array_equal(st["top_sums"], ref.top_sums)
     pos = ref.top_sums > 0
     np.testing.assert_array_equal(st["top_blocks"][pos], ref.top_blocks[pos])
