@@ -231,6 +231,122 @@ sc_status sc_writer_record_hit(sc_writer* w, const sc_hit* hit, const float* inp
 /* Closes the file (and renames .hits.tmp to .hits). */
 sc_status sc_writer_close(sc_writer* w);
 
+/* ---- Learned block proposal ------------------------------------------------
+ * No counterpart in the reference. A small BF16 convolutional network scores
+ * 4096-channel blocks of a 16-row coarse channel from a pooled tile
+ * (16 x 1024, 4 fine channels per column, normalised by the coarse channel's
+ * column-sum median and deviation); the gated search runs the full tree only
+ * on the channel tiles of blocks whose logit exceeds a threshold. The plain
+ * search is its fallback and the judge of its recall. */
+
+#define SC_PN_BLOCK_CHANNELS 4096
+#define SC_PN_TILE_ROWS 16
+#define SC_PN_TILE_COLUMNS 1024
+
+typedef struct sc_pn sc_pn;
+
+/* Number of FP32 parameters (23393): conv1 1->16, conv2 16->32, conv3 32->32
+ * (3x5 kernels), 1x1 head 32->1; the layout is documented in pn_internal.h. */
+int sc_pn_num_params(void);
+
+/* Allocates weights, optimizer state and activation buffers for up to
+ * max_batch blocks per call on the current device. Parameters start at 0. */
+sc_status sc_pn_create(int max_batch, sc_pn** out);
+sc_status sc_pn_destroy(sc_pn* pn);
+sc_status sc_pn_memory_usage(const sc_pn* pn, size_t* bytes);
+
+/* He-uniform weights and zero biases from a seed (deterministic). */
+sc_status sc_pn_init(sc_pn* pn, uint64_t seed);
+/* Host FP32 parameters in/out (n == sc_pn_num_params()); set resets Adam. */
+sc_status sc_pn_set_params(sc_pn* pn, const float* params, size_t n);
+sc_status sc_pn_get_params(const sc_pn* pn, float* params, size_t n);
+
+/* Pooled tiles (device BF16 [N/4096][16][1024]) of a device [16][N] coarse
+ * channel; stats = device {median, std} of its column sums. */
+sc_status sc_pn_pool(const void* data, int dtype, int rounded_timesteps, int num_channels,
+                     long row_stride, const float* stats, void* tiles, void* stream);
+
+/* Block logits (device float[num_blocks]) of device tiles. */
+sc_status sc_pn_forward(sc_pn* pn, const void* tiles, int num_blocks, float* logits,
+                        void* stream);
+
+/* Gradient (device float[sc_pn_num_params()], or the handle's own buffer if
+ * NULL) of the mean weighted binary cross-entropy of the block logits against
+ * device labels (0/1 floats; positives weighted by pos_weight), and the loss
+ * (device float, may be NULL). */
+sc_status sc_pn_gradient(sc_pn* pn, const void* tiles, const float* labels, int num_blocks,
+                         float pos_weight, float* grad, float* loss, void* stream);
+
+/* sc_pn_gradient followed by one Adam step (beta 0.9/0.999) at rate lr. */
+sc_status sc_pn_train_step(sc_pn* pn, const void* tiles, const float* labels, int num_blocks,
+                           float lr, float pos_weight, float* loss, void* stream);
+
+/* Device pointers of the last forward's activations (BF16 NHWC h1 [B][16][512][16],
+ * h2 [B][8][256][32], h3 [B][4][128][32]) and of the last gradient call's logits
+ * and argmax positions; any output may be NULL. For tests. */
+sc_status sc_pn_activations(const sc_pn* pn, const void** h1, const void** h2, const void** h3,
+                            const float** logits, const int** argmax);
+
+/* Gated form of sc_dd_search (rounded_timesteps 16, num_channels a multiple of
+ * 4096): device statistics, then the network, then the full tree only on the
+ * channel tiles that overlap a block with logit > logit_threshold. Hits are
+ * those the plain device-statistics search reports whose start channel lies in
+ * a searched tile (unsearched channels neither form nor suppress candidates).
+ * flagged_blocks (may be NULL) receives the number of flagged blocks. */
+sc_status sc_dd_search_proposed(sc_dd* dd, sc_pn* pn, const void* data, int dtype,
+                                int rounded_timesteps, int num_channels, long row_stride,
+                                const sc_meta* meta, int beam, int coarse_channel,
+                                double max_drift, double min_drift, double snr_threshold,
+                                float logit_threshold, sc_hit* hits, size_t capacity,
+                                size_t* num_hits, int* flagged_blocks, void* stream);
+
+/* ---- Batched search --------------------------------------------------------
+ * No counterpart in the reference, which searches one coarse channel per call
+ * (run_dedoppler.cpp:35-51, beamforming_pipeline.cpp:145-253). A batch of
+ * coarse channels / beams with uneven time-step counts and channel widths is
+ * planned once (one search handle per distinct shape, each instance's launch
+ * sequence captured into one hipGraph) and then run as a single graph launch.
+ * Statistics are the device statistics (SC_STATS_DEVICE). */
+
+typedef struct {
+  const void* data;          /* device [rounded_timesteps][row_stride] samples */
+  int dtype;                 /* sc_dtype */
+  int num_timesteps;         /* unrounded */
+  int num_channels;          /* >= 4096 */
+  long row_stride;
+  double foff;               /* MHz */
+  double tsamp;              /* s */
+  sc_meta meta;
+  int beam;
+  int coarse_channel;
+  double max_drift, min_drift, snr_threshold;
+} sc_instance;
+
+/* Kernel shape for one (rounded_timesteps, num_channels): the fused-kernel
+ * fields apply when rounded_timesteps <= 16, the two-pass fields for 32..256
+ * (see sc_dd_set_tuning / sc_dd_set_two_pass). Produced by the autotuner. */
+typedef struct {
+  int rounded_timesteps;
+  int num_channels;
+  int channels_per_lane, drift_group, prefetch;
+  int pass_group, pass_blocks;
+} sc_tuning;
+
+typedef struct sc_batch sc_batch;
+
+/* Plans the batch: instance data pointers are captured, so the same buffers
+ * must hold each run's input. tuning may be NULL (default shapes). */
+sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuning* tuning,
+                          int num_tuning, sc_batch** out);
+/* Runs every instance (one graph launch on `stream`), waits, and extracts the hits. */
+sc_status sc_batch_run(sc_batch* b, void* stream);
+/* Hits of one instance from the last run (same contract as sc_dd_copy_hits). */
+sc_status sc_batch_copy_hits(const sc_batch* b, int instance, sc_hit* hits, size_t capacity,
+                             size_t* num_hits);
+/* Number of distinct shapes and of graph nodes. */
+sc_status sc_batch_info(const sc_batch* b, int* num_shapes, int* graph_nodes);
+sc_status sc_batch_destroy(sc_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

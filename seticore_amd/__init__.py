@@ -133,6 +133,19 @@ def lib():
                                 P(d), P(d), P(vp)],
         "sc_writer_record_hit": [vp, P(ScHit), P(ctypes.c_float)],
         "sc_writer_close": [vp],
+        "sc_pn_create": [i, P(vp)],
+        "sc_pn_destroy": [vp],
+        "sc_pn_memory_usage": [vp, P(sz)],
+        "sc_pn_init": [vp, ctypes.c_uint64],
+        "sc_pn_set_params": [vp, P(ctypes.c_float), sz],
+        "sc_pn_get_params": [vp, P(ctypes.c_float), sz],
+        "sc_pn_pool": [vp, i, i, i, lng, vp, vp, vp],
+        "sc_pn_forward": [vp, vp, i, vp, vp],
+        "sc_pn_gradient": [vp, vp, vp, i, ctypes.c_float, vp, vp, vp],
+        "sc_pn_train_step": [vp, vp, vp, i, ctypes.c_float, ctypes.c_float, vp, vp],
+        "sc_pn_activations": [vp, P(vp), P(vp), P(vp), P(vp), P(vp)],
+        "sc_dd_search_proposed": [vp, vp, vp, i, i, i, lng, P(ScMeta), i, i, d, d, d,
+                                  ctypes.c_float, P(ScHit), sz, P(sz), P(i), vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -142,6 +155,8 @@ def lib():
     L.sc_last_error_string.restype = ctypes.c_char_p
     L.sc_abi_version.argtypes = []
     L.sc_abi_version.restype = ctypes.c_int
+    L.sc_pn_num_params.argtypes = []
+    L.sc_pn_num_params.restype = ctypes.c_int
     _lib_handle = L
     return L
 
@@ -358,6 +373,32 @@ class Dedopplerer:
         output.extend(new)
         return new
 
+    def search_proposed(self, input, net, metadata, beam, coarse_channel, max_drift, min_drift,
+                        snr_threshold, logit_threshold, output, stream=None):
+        """Gated search: the full tree only on blocks the proposal network flags
+        (device statistics). Appends hits to `output`; returns the number of
+        flagged 4096-channel blocks."""
+        L = lib()
+        n = self._n
+        flagged = ctypes.c_int()
+        status = L.sc_dd_search_proposed(self._h, net._h, input.data.data_ptr(), input.dtype,
+                                         input.num_timesteps, input.num_channels,
+                                         input.row_stride, ctypes.byref(metadata._sc_meta()),
+                                         beam, coarse_channel, max_drift, min_drift,
+                                         snr_threshold, float(logit_threshold), self._hits,
+                                         self._cap, ctypes.byref(n), ctypes.byref(flagged),
+                                         _stream_ptr(stream))
+        if status != SC_OK:
+            check(status)
+        count = n.value
+        if count > self._cap:
+            self._cap = count
+            self._hits = (ScHit * count)()
+            check(L.sc_dd_copy_hits(self._h, self._hits, count, ctypes.byref(n)))
+        new = list((ScHit * count).from_buffer_copy(self._hits)) if count else []
+        output.extend(new)
+        return flagged.value
+
     def state(self):
         """Column sums, top sums/blocks/offsets, median and std of the last search."""
         import numpy as np
@@ -504,3 +545,117 @@ def normalize(data, out_dtype, clamp=65504.0, stream=None):
                              ctypes.c_void_p(out.data_ptr()), ctypes.byref(mean),
                              ctypes.byref(sd), _stream_ptr(stream)))
     return out, mean.value, sd.value
+
+
+# ---------------------------------------------------------------------------
+# Learned block proposal network (no counterpart in the reference).
+
+BLOCK_CHANNELS = 4096
+TILE_ROWS = 16
+TILE_COLUMNS = 1024
+
+
+def pn_num_params():
+    return lib().sc_pn_num_params()
+
+
+class ProposalNet:
+    """The block proposal network: BF16 weights/activations, FP32 master
+    weights and Adam state on the device. Tiles are BF16 CUDA tensors
+    [B][16][1024]; labels and logits float32 CUDA tensors [B]."""
+
+    def __init__(self, max_batch=256, seed=None):
+        h = ctypes.c_void_p()
+        check(lib().sc_pn_create(int(max_batch), ctypes.byref(h)))
+        self._h = h
+        self.max_batch = int(max_batch)
+        if seed is not None:
+            self.init(seed)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib_handle is not None:
+            _lib_handle.sc_pn_destroy(h)
+            self._h = None
+
+    def init(self, seed):
+        check(lib().sc_pn_init(self._h, ctypes.c_uint64(seed)))
+
+    def get_params(self):
+        import numpy as np
+        p = np.empty(pn_num_params(), np.float32)
+        check(lib().sc_pn_get_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                     p.size))
+        return p
+
+    def set_params(self, params):
+        import numpy as np
+        p = np.ascontiguousarray(params, np.float32)
+        check(lib().sc_pn_set_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                     p.size))
+
+    def memoryUsage(self):
+        n = ctypes.c_size_t()
+        check(lib().sc_pn_memory_usage(self._h, ctypes.byref(n)))
+        return n.value
+
+    @staticmethod
+    def pool(data, stats, dtype=SC_F32, out=None, stream=None):
+        """BF16 tiles [N/4096][16][1024] of a CUDA [16][N] coarse channel;
+        stats = CUDA float32 tensor {median, std} of its column sums."""
+        torch = _torch()
+        T, N = data.shape
+        if out is None:
+            out = torch.empty((N // BLOCK_CHANNELS, TILE_ROWS, TILE_COLUMNS), dtype=torch.bfloat16,
+                              device=data.device)
+        check(lib().sc_pn_pool(data.data_ptr(), int(dtype), T, N, data.stride(0),
+                               stats.data_ptr(), out.data_ptr(), _stream_ptr(stream)))
+        return out
+
+    def forward(self, tiles, stream=None):
+        torch = _torch()
+        B = tiles.shape[0]
+        logits = torch.empty(B, dtype=torch.float32, device=tiles.device)
+        check(lib().sc_pn_forward(self._h, tiles.data_ptr(), B, logits.data_ptr(),
+                                  _stream_ptr(stream)))
+        return logits
+
+    def gradient(self, tiles, labels, pos_weight=1.0, stream=None):
+        """(grad tensor [num_params], loss tensor [1])."""
+        torch = _torch()
+        grad = torch.empty(pn_num_params(), dtype=torch.float32, device=tiles.device)
+        loss = torch.empty(1, dtype=torch.float32, device=tiles.device)
+        check(lib().sc_pn_gradient(self._h, tiles.data_ptr(), labels.data_ptr(), tiles.shape[0],
+                                   float(pos_weight), grad.data_ptr(), loss.data_ptr(),
+                                   _stream_ptr(stream)))
+        return grad, loss
+
+    def train_step(self, tiles, labels, lr, pos_weight=1.0, loss=None, stream=None):
+        check(lib().sc_pn_train_step(self._h, tiles.data_ptr(), labels.data_ptr(),
+                                     tiles.shape[0], float(lr), float(pos_weight),
+                                     loss.data_ptr() if loss is not None else None,
+                                     _stream_ptr(stream)))
+
+    def activations(self, B):
+        """Views of the last forward's activations and the last gradient call's
+        logits/argmax: (h1, h2, h3, logits, argmax) as CUDA tensors."""
+        torch = _torch()
+        ptrs = [ctypes.c_void_p() for _ in range(5)]
+        check(lib().sc_pn_activations(self._h, *[ctypes.byref(p) for p in ptrs]))
+        shapes = [(B, 16, 512, 16), (B, 8, 256, 32), (B, 4, 128, 32), (B,), (B,)]
+        dtypes = [torch.bfloat16] * 3 + [torch.float32, torch.int32]
+        return [_device_view(p.value, sh, dt) for p, sh, dt in zip(ptrs, shapes, dtypes)]
+
+
+def _device_view(ptr, shape, dtype):
+    """A CUDA tensor viewing device memory owned by the library (copied)."""
+    torch = _torch()
+    out = torch.empty(shape, dtype=dtype, device="cuda")
+    n = out.numel() * out.element_size()
+    torch.cuda.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+    if hip.hipMemcpy(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ptr), n, 3) != 0:
+        raise SeticoreError(SC_ERR_HIP, "hipMemcpy of activations failed")
+    return out

@@ -27,7 +27,9 @@ DEVICE = ["--offload-arch=" + ARCH, "-fno-honor-nans",
 
 SOURCES = [
     ("dd_kernels.hip", DEVICE),
+    ("pn_kernels.hip", DEVICE),
     ("dd_capi.cpp", []),
+    ("pn_capi.cpp", []),
     ("hit_writers.cpp", []),
 ]
 

@@ -19,6 +19,8 @@
 
 #include "../../include/seticore_amd.h"
 #include "dd_internal.h"
+#include "pn_internal.h"
+#include "pn_state.h"
 #include "sc_error.h"
 
 namespace sc {
@@ -99,6 +101,10 @@ struct sc_dd {
   int pass_group = 4;
   int pass_blocks = 0;  // drift blocks per launch pair; 0 = from the size
   int pass_alloc_blocks = 0;
+  // Gated search (learned block proposal): channel-tile list and counters.
+  int* gate_list = nullptr;    // one entry per channel tile
+  int* gate_count = nullptr;   // {tile count, flagged blocks}
+  int* h_gate = nullptr;       // pinned mirror of gate_count
   int* gather_idx = nullptr;   // 2*capacity ints
   float* gather_out = nullptr;
   size_t gather_capacity = 0;
@@ -176,6 +182,9 @@ void release(sc_dd* dd) {
   hipFree(dd->tree_a);
   hipFree(dd->tree_b);
   hipFree(dd->pass_scratch);
+  hipFree(dd->gate_list);
+  hipFree(dd->gate_count);
+  hipHostFree(dd->h_gate);
   hipFree(dd->gather_idx);
   hipFree(dd->gather_out);
   hipHostFree(dd->h_colsum);
@@ -530,7 +539,10 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
 
 // Device statistics, window merge and the copies of the per-window candidates.
 sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_threshold,
-                              hipStream_t stream) {
+                              hipStream_t stream, int4* h_cand = nullptr,
+                              float* h_stats = nullptr) {
+  if (!h_cand) h_cand = dd->h_cand;
+  if (!h_stats) h_stats = dd->h_stats;
   const int T = dd->rounded, N = dd->num_channels;
   SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
                                  dd->stats_out, stream),
@@ -540,9 +552,9 @@ sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_thresho
                                  stream),
          "window merge");
   const int nwin = (int)(((long)N + r.window - 1) / r.window);
-  SC_HIP(hipMemcpyAsync(dd->h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost,
-                        stream), "copy candidates");
-  SC_HIP(hipMemcpyAsync(dd->h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
+  SC_HIP(hipMemcpyAsync(h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost, stream),
+         "copy candidates");
+  SC_HIP(hipMemcpyAsync(h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
                         stream), "copy stats");
   return SC_OK;
 }
@@ -590,14 +602,20 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
 
 // Hits from the per-window candidates (dedoppler.cu:305-317).
 void device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
-                 int coarse_channel, double min_drift) {
+                 int coarse_channel, double min_drift, const int4* h_cand = nullptr,
+                 const float* h_stats = nullptr, std::vector<sc_hit>* out = nullptr,
+                 int num_timesteps = 0) {
+  if (!h_cand) h_cand = dd->h_cand;
+  if (!h_stats) h_stats = dd->h_stats;
+  if (!out) out = &dd->hits;
+  if (num_timesteps <= 0) num_timesteps = dd->num_timesteps;
   const int N = dd->num_channels;
-  const float median = dd->h_stats[0], std_dev = dd->h_stats[1];
+  const float median = h_stats[0], std_dev = h_stats[1];
   dd->last_median = median;
   dd->last_std = std_dev;
   const int nwin = (int)(((long)N + r.window - 1) / r.window);
   for (int i = 0; i < nwin; ++i) {
-    const int4 c = dd->h_cand[i];
+    const int4 c = h_cand[i];
     if (c.x < 0) continue;
     float cand_sum;
     std::memcpy(&cand_sum, &c.y, sizeof(float));
@@ -605,8 +623,8 @@ void device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
     double drift_rate = drift_bins * dd->resolution;
     float snr = (cand_sum - median) / std_dev;
     if (std::fabs(drift_rate) >= min_drift)
-      dd->hits.push_back(make_hit(meta, c.x, drift_bins, drift_rate, snr, beam, coarse_channel,
-                                  dd->num_timesteps, cand_sum));
+      out->push_back(make_hit(meta, c.x, drift_bins, drift_rate, snr, beam, coarse_channel,
+                              num_timesteps, cand_sum));
   }
 }
 
@@ -681,6 +699,97 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
     dd->kernel_launches += 1;
   }
   dd->state_valid = true;
+  if (num_hits) *num_hits = dd->hits.size();
+  if (hits && capacity) {
+    size_t n = std::min(capacity, dd->hits.size());
+    std::memcpy(hits, dd->hits.data(), n * sizeof(sc_hit));
+  }
+  return SC_OK;
+}
+
+sc_status sc_dd_search_proposed(sc_dd* dd, sc_pn* pn, const void* data, int dtype,
+                                int rounded_timesteps, int num_channels, long row_stride,
+                                const sc_meta* meta, int beam, int coarse_channel,
+                                double max_drift, double min_drift, double snr_threshold,
+                                float logit_threshold, sc_hit* hits, size_t capacity,
+                                size_t* num_hits, int* flagged_blocks, void* stream_ptr) {
+  sc_status s = validate_input(dd, data, dtype, rounded_timesteps, num_channels, row_stride);
+  if (s != SC_OK) return s;
+  if (!pn) return fail(SC_ERR_INVALID_ARGUMENT, "null proposal network");
+  if (!meta) return fail(SC_ERR_INVALID_ARGUMENT, "null metadata");
+  if (!(max_drift > 0) || !std::isfinite(max_drift))
+    return fail(SC_ERR_INVALID_ARGUMENT,
+                "max_drift must be > 0 (the reference loops forever at 0, dedoppler.cu:275)");
+  const int T = dd->rounded, N = dd->num_channels;
+  if (T != sc::pn::kRows || N % sc::pn::kBlockChannels != 0)
+    return fail(SC_ERR_UNSUPPORTED,
+                "the gated search takes 16 time steps and a multiple of 4096 channels");
+  const int nblk = N / sc::pn::kBlockChannels;
+  if (nblk > pn->max_batch)
+    return fail(SC_ERR_INVALID_ARGUMENT, "proposal network max_batch below the block count");
+  if (!sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch))
+    return fail(SC_ERR_UNSUPPORTED, "no fused kernel for the current tuning");
+  hipStream_t stream = (hipStream_t)stream_ptr;
+  const DriftRange r = drift_range(dd, max_drift);
+  if (r.window <= 0) return fail(SC_ERR_INVALID_ARGUMENT, "window size is 0");
+  const int nblocks = r.bmax - r.bmin + 1;
+  if ((long)nblocks * T > 0x1FFFFFFFL) return fail(SC_ERR_UNSUPPORTED, "drift range too large");
+  const int width = sc::fused_tile_width(T, dd->lanes);
+  const int ntiles = (N + width - 1) / width;
+  if (!dd->gate_list) {
+    s = dev_alloc(dd, (void**)&dd->gate_list, sizeof(int) * (size_t)((N + 63) / 64 + 1),
+                  "tile list");
+    if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->gate_count, 2 * sizeof(int), "tile count");
+    if (s == SC_OK) s = host_alloc((void**)&dd->h_gate, 2 * sizeof(int), "tile count");
+    if (s != SC_OK) return s;
+  }
+  if (ntiles > (N + 63) / 64 + 1) return fail(SC_ERR_UNSUPPORTED, "tile list too long");
+  dd->hits.clear();
+  dd->state_valid = false;
+
+  // Column sums and statistics of the whole coarse channel first (they do
+  // not depend on the search), then the network, then the full tree on the
+  // flagged blocks' channel tiles only; unsearched channels keep key 0 (no
+  // path), so they never form or suppress a candidate.
+  SC_HIP(sc::launch_reset(dd->top_key, N, dd->stats_hist, stream), "reset");
+  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_start, stream), "timing event");
+  SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
+  SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
+                                 dd->stats_out, stream),
+         "device stats");
+  SC_HIP(sc::pn::launch_pool(data, dtype, row_stride, nblk, dd->stats_out, pn->z, stream),
+         "pool");
+  SC_HIP(sc::pn::launch_forward(pn->w, pn->z, nblk, pn->a, pn->logits, nullptr, stream),
+         "proposal forward");
+  SC_HIP(sc::pn::launch_tile_list(pn->logits, nblk, logit_threshold, N, width, dd->gate_list,
+                                  dd->gate_count, dd->gate_count + 1, stream),
+         "tile list");
+  SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
+                                   dd->group, dd->prefetch, dd->top_key, nullptr, stream,
+                                   dd->gate_list, dd->gate_count),
+         "gated tree");
+  SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
+                                 dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off, dd->cand,
+                                 stream),
+         "window merge");
+  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
+  const int nwin = (int)(((long)N + r.window - 1) / r.window);
+  SC_HIP(hipMemcpyAsync(dd->h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost,
+                        stream), "copy candidates");
+  SC_HIP(hipMemcpyAsync(dd->h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
+                        stream), "copy stats");
+  SC_HIP(hipMemcpyAsync(dd->h_gate, dd->gate_count, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                        stream), "copy tile count");
+  SC_HIP(hipStreamSynchronize(stream), "gated search");
+  device_hits(dd, r, meta, beam, coarse_channel, min_drift);
+  if (dd->timing) {
+    float ms = 0.0f;
+    SC_HIP(hipEventElapsedTime(&ms, dd->ev_start, dd->ev_stop), "timing");
+    dd->kernel_ms += ms;
+    dd->kernel_launches += 1;
+  }
+  dd->state_valid = true;
+  if (flagged_blocks) *flagged_blocks = dd->h_gate[1];
   if (num_hits) *num_hits = dd->hits.size();
   if (hits && capacity) {
     size_t n = std::min(capacity, dd->hits.size());
@@ -876,6 +985,188 @@ sc_status sc_gen_waterfall(float* out, int num_timesteps, int rounded_timesteps,
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   hipFree(dev);
   if (e != hipSuccess) return hip_fail(e, "generate waterfall");
+  return SC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched search: many coarse channels / beams of uneven shapes in one graph.
+//
+// The reference searches one coarse channel per call (run_dedoppler.cpp:35-51,
+// beamforming_pipeline.cpp:145-253). Here every instance's launch sequence
+// (reset, path sums, device statistics, window merge, candidate copy) is
+// captured once into a single hipGraph and the whole batch runs as one graph
+// launch. Instances of one shape (rounded time steps, channels, foff, tsamp)
+// share a search handle - their device state is reused in sequence, and each
+// instance's candidates land in its own pinned host area.
+
+struct sc_batch {
+  struct Instance {
+    sc_instance in;
+    DriftRange r;
+    int shape = 0;
+    size_t cand_offset = 0;  // into h_cand
+    std::vector<sc_hit> hits;
+  };
+  std::vector<Instance> inst;
+  std::vector<sc_dd*> shapes;
+  int4* h_cand = nullptr;
+  float* h_stats = nullptr;  // 2 per instance
+  hipStream_t capture_stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int nodes = 0;
+};
+
+namespace {
+
+void batch_release(sc_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->capture_stream) hipStreamDestroy(b->capture_stream);
+  for (sc_dd* dd : b->shapes) sc_dd_destroy(dd);
+  hipHostFree(b->h_cand);
+  hipHostFree(b->h_stats);
+}
+
+}  // namespace
+
+extern "C" {
+
+sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuning* tuning,
+                          int num_tuning, sc_batch** out) {
+  if (!out) return fail(SC_ERR_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  if (!instances || count <= 0) return fail(SC_ERR_INVALID_ARGUMENT, "empty batch");
+  if (num_tuning > 0 && !tuning) return fail(SC_ERR_INVALID_ARGUMENT, "null tuning table");
+  sc_batch* b = new sc_batch();
+  sc_status s = SC_OK;
+  size_t total_windows = 0;
+  for (int i = 0; i < count && s == SC_OK; ++i) {
+    const sc_instance& in = instances[i];
+    sc_batch::Instance bi;
+    bi.in = in;
+    if (!in.data || in.num_channels < 4096 || in.num_timesteps <= 1 ||
+        !(in.max_drift > 0) || !std::isfinite(in.max_drift) || in.row_stride < in.num_channels) {
+      s = fail(SC_ERR_INVALID_ARGUMENT,
+               "instance " + std::to_string(i) +
+                   ": needs data, >= 4096 channels, > 1 time step, max_drift > 0");
+      break;
+    }
+    const int T = round_up_pow2(in.num_timesteps);
+    int shape = -1;
+    for (size_t k = 0; k < b->shapes.size(); ++k) {
+      const sc_dd* dd = b->shapes[k];
+      if (dd->rounded == T && dd->num_channels == in.num_channels && dd->foff == in.foff &&
+          dd->tsamp == in.tsamp)
+        shape = (int)k;
+    }
+    if (shape < 0) {
+      sc_dd* dd = nullptr;
+      s = sc_dd_create(in.num_timesteps, in.num_channels, in.foff, in.tsamp, 0, &dd);
+      if (s != SC_OK) break;
+      dd->stats_mode = SC_STATS_DEVICE;
+      b->shapes.push_back(dd);
+      shape = (int)b->shapes.size() - 1;
+      for (int k = 0; k < num_tuning; ++k) {
+        const sc_tuning& t = tuning[k];
+        if (t.rounded_timesteps != T || t.num_channels != in.num_channels) continue;
+        if (T <= 16)
+          s = sc_dd_set_tuning(dd, t.channels_per_lane, t.drift_group, t.prefetch);
+        else if (sc::two_pass_supported(T, t.pass_group))
+          s = sc_dd_set_two_pass(dd, t.pass_group, t.pass_blocks);
+        break;
+      }
+      if (s != SC_OK) break;
+    }
+    bi.shape = shape;
+    bi.r = drift_range(b->shapes[shape], in.max_drift);
+    if (bi.r.window <= 0 || (long)(bi.r.bmax - bi.r.bmin + 1) * T > 0x1FFFFFFFL) {
+      s = fail(SC_ERR_UNSUPPORTED, "instance " + std::to_string(i) + ": drift range");
+      break;
+    }
+    bi.cand_offset = total_windows;
+    total_windows += ((size_t)in.num_channels + bi.r.window - 1) / bi.r.window;
+    b->inst.push_back(bi);
+  }
+  if (s == SC_OK) s = host_alloc((void**)&b->h_cand, total_windows * sizeof(int4), "candidates");
+  if (s == SC_OK)
+    s = host_alloc((void**)&b->h_stats, 2 * sizeof(float) * (size_t)count, "statistics");
+  if (s == SC_OK) {
+    hipError_t e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) s = hip_fail(e, "stream");
+  }
+  if (s == SC_OK) {
+    hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) s = hip_fail(e, "begin capture");
+    for (size_t i = 0; i < b->inst.size() && s == SC_OK && e == hipSuccess; ++i) {
+      sc_batch::Instance& bi = b->inst[i];
+      sc_dd* dd = b->shapes[bi.shape];
+      s = enqueue_path_sums(dd, bi.in.data, bi.in.dtype, bi.in.row_stride, bi.r,
+                            b->capture_stream);
+      if (s == SC_OK)
+        s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, b->capture_stream,
+                                b->h_cand + bi.cand_offset, b->h_stats + 2 * i);
+    }
+    hipGraph_t graph = nullptr;
+    if (e == hipSuccess) {
+      hipError_t e2 = hipStreamEndCapture(b->capture_stream, &graph);
+      if (s == SC_OK && e2 != hipSuccess) s = hip_fail(e2, "end capture");
+    }
+    if (s == SC_OK) {
+      size_t n = 0;
+      hipGraphGetNodes(graph, nullptr, &n);
+      b->nodes = (int)n;
+      e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) s = hip_fail(e, "instantiate graph");
+    }
+    if (graph) hipGraphDestroy(graph);
+  }
+  if (s != SC_OK) {
+    batch_release(b);
+    delete b;
+    return s;
+  }
+  *out = b;
+  return SC_OK;
+}
+
+sc_status sc_batch_run(sc_batch* b, void* stream_ptr) {
+  if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
+  hipStream_t stream = (hipStream_t)stream_ptr;
+  SC_HIP(hipGraphLaunch(b->exec, stream), "batch graph launch");
+  SC_HIP(hipStreamSynchronize(stream), "batch");
+  for (size_t i = 0; i < b->inst.size(); ++i) {
+    sc_batch::Instance& bi = b->inst[i];
+    bi.hits.clear();
+    device_hits(b->shapes[bi.shape], bi.r, &bi.in.meta, bi.in.beam, bi.in.coarse_channel,
+                bi.in.min_drift, b->h_cand + bi.cand_offset, b->h_stats + 2 * i, &bi.hits,
+                bi.in.num_timesteps);
+  }
+  return SC_OK;
+}
+
+sc_status sc_batch_copy_hits(const sc_batch* b, int instance, sc_hit* hits, size_t capacity,
+                             size_t* num_hits) {
+  if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
+  if (instance < 0 || instance >= (int)b->inst.size())
+    return fail(SC_ERR_INVALID_ARGUMENT, "instance out of range");
+  const std::vector<sc_hit>& h = b->inst[instance].hits;
+  if (num_hits) *num_hits = h.size();
+  if (hits && capacity) std::memcpy(hits, h.data(), std::min(capacity, h.size()) * sizeof(sc_hit));
+  return SC_OK;
+}
+
+sc_status sc_batch_info(const sc_batch* b, int* num_shapes, int* graph_nodes) {
+  if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
+  if (num_shapes) *num_shapes = (int)b->shapes.size();
+  if (graph_nodes) *graph_nodes = b->nodes;
+  return SC_OK;
+}
+
+sc_status sc_batch_destroy(sc_batch* b) {
+  if (!b) return SC_OK;
+  batch_release(b);
+  delete b;
   return SC_OK;
 }
 

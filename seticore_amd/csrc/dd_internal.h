@@ -18,11 +18,16 @@ typedef unsigned long long top_key_t;
 
 // Fused tree + top-path tracking for T_r <= 16 over all drift blocks in one
 // launch. C = channels per lane, G = drift blocks per LDS window, P = 1 to
-// prefetch the next window into registers. Writes column sums for every
-// channel and max-merges top keys.
+// prefetch the next window into registers. Writes column sums (unless colsum
+// is null) and max-merges top keys. With tile_list (device), only the channel
+// tiles tile_list[0 .. *tile_count) are searched (tiles of
+// fused_tile_width(T, C) channels).
 hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, int N,
                                  int bmin, int nblocks, int C, int G, int P,
-                                 top_key_t* top_key, float* colsum, hipStream_t stream);
+                                 top_key_t* top_key, float* colsum, hipStream_t stream,
+                                 const int* tile_list = nullptr,
+                                 const int* tile_count = nullptr);
+int fused_tile_width(int T, int C);
 
 // Whether launch_fused_tree_top has a kernel for (T, C, G, P).
 bool fused_supported(int T, int C, int G, int P);

@@ -302,17 +302,22 @@ __global__ void __launch_bounds__(kWave)
     fused_tree_top_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
                           int ngroups, int chunks, int groups_per_chunk, int total_items,
                           unsigned long long* __restrict__ top_key,
-                          float* __restrict__ colsum) {
+                          float* __restrict__ colsum, const int* __restrict__ tile_list,
+                          const int* __restrict__ tile_count) {
   using Sh = TileShape<T, C>;
   using Wn = Window<T, C, G>;
   __shared__ float win_storage[Wn::kFloats];
   lds_float* win = (lds_float*)win_storage;
 
-  const int per_xcd = gridDim.x / 8;
+  // With a tile list (gated search) only its first *tile_count tiles are
+  // searched; the active items are dealt over the XCDs the same way.
+  const int active = tile_list ? min(total_items, *tile_count * chunks) : total_items;
+  const int per_xcd = tile_list ? (active + 7) / 8 : gridDim.x / 8;
+  if ((int)blockIdx.x / 8 >= per_xcd) return;
   const int item = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  if (item >= total_items) return;
-  const int tile = item / chunks;
-  const int chunk = item - tile * chunks;
+  if (item >= active) return;
+  const int tile = tile_list ? tile_list[item / chunks] : item / chunks;
+  const int chunk = item % chunks;
   const int c_w = tile * Sh::kOut;
   const int g_begin = chunk * groups_per_chunk;
   const int g_end = min(ngroups, g_begin + groups_per_chunk);
@@ -322,7 +327,7 @@ __global__ void __launch_bounds__(kWave)
   fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + g_begin * G, 0);
   const int b_first = bmin + g_begin * G;
   const int b_last = min(bmin + g_end * G, bmin + nblocks) - 1;
-  if (b_first <= 0 && 0 <= b_last && lane < Sh::kValidLanes) {
+  if (colsum && b_first <= 0 && 0 <= b_last && lane < Sh::kValidLanes) {
 #pragma unroll
     for (int j = 0; j < C; ++j) {
       const int c = c_w + lane * C + j;
@@ -426,7 +431,8 @@ __global__ void __launch_bounds__(kWave)
 // paths (dedoppler.cu:40-46) and folds the keys.
 //
 // Scratch layout: row ((gi*16 + q)*M + k) of row_pitch floats holds R_q[k] of
-// drift block b_first + gi. Entries that no valid path reads are not written.
+// drift block b_first + gi, starting (k*q) & 3 floats in. Entries that no
+// valid path reads are not written.
 
 template <int C, int G, int gp>
 __device__ __forceinline__ void level16_block(lds_float* const (&rp)[16], int g, int ng, int b0,
@@ -444,13 +450,14 @@ __device__ __forceinline__ void level16_block(lds_float* const (&rp)[16], int g,
         float* base = scratch + (long)((gi0 + g + gp) * 16 * M + k) * row_pitch;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-          const long col = (long)c - (long)k * (16 * b + q);
+          // Stored (k*q) & 3 columns to the right of its place, so that every
+          // lane's store is 16-byte aligned (c and 16*k*b are multiples of 4);
+          // the second pass reads with that offset.
+          const long col = (long)c - (long)k * (16 * b + q) + ((k * q) & 3);
           float* dst = base + (long)q * M * row_pitch;
           if (col >= 0 && col + C <= row_pitch) {
             if constexpr (C == 4) {
-              typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
-              f4u w = {v[q][0], v[q][1], v[q][2], v[q][3]};
-              *(f4u*)(dst + col) = w;
+              *(float4*)(dst + col) = make_float4(v[q][0], v[q][1], v[q][2], v[q][3]);
             } else {
 #pragma unroll
               for (int j = 0; j < C; ++j) dst[col + j] = v[q][j];
@@ -530,14 +537,15 @@ __global__ void __launch_bounds__(kWave * kUpperWaves)
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       if constexpr (C == 4) {
-        const float4 w = *(const float4*)(src + (long)k * row_pitch);
+        typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+        const f4u w = *(const f4u*)(src + (long)k * row_pitch + ((k * q) & 3));
         v[k][0] = w.x;
         v[k][1] = w.y;
         v[k][2] = w.z;
         v[k][3] = w.w;
       } else {
 #pragma unroll
-        for (int j = 0; j < C; ++j) v[k][j] = src[(long)k * row_pitch + j];
+        for (int j = 0; j < C; ++j) v[k][j] = src[(long)k * row_pitch + ((k * q) & 3) + j];
       }
     }
     tree_regs<M, C>(v);
@@ -1171,7 +1179,8 @@ inline int blocks_for(long n, int threads, int cap) {
 
 template <int T, int C, int G, bool PF, typename In>
 hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nblocks,
-                              top_key_t* top_key, float* colsum, hipStream_t stream) {
+                              top_key_t* top_key, float* colsum, const int* tile_list,
+                              const int* tile_count, hipStream_t stream) {
   using Sh = TileShape<T, C>;
   const long ntiles = ((long)N + Sh::kOut - 1) / Sh::kOut;
   const long ngroups = ((long)nblocks + G - 1) / G;
@@ -1189,7 +1198,7 @@ hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nbl
   const int grid = (int)((total + 7) / 8 * 8);
   hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, PF, In>), dim3(grid), dim3(kWave), 0,
                      stream, x, stride, N, bmin, nblocks, (int)ngroups, (int)chunks,
-                     (int)per_chunk, (int)total, top_key, colsum);
+                     (int)per_chunk, (int)total, top_key, colsum, tile_list, tile_count);
   return hipGetLastError();
 }
 
@@ -1209,11 +1218,12 @@ hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nbl
 
 template <typename In>
 hipError_t fused_dispatch(const In* x, long stride, int T, int N, int bmin, int nblocks, int C,
-                          int G, int P, top_key_t* top_key, float* colsum, hipStream_t stream) {
+                          int G, int P, top_key_t* top_key, float* colsum, const int* tile_list,
+                          const int* tile_count, hipStream_t stream) {
 #define SC_FUSED_CASE(TT, CC, GG, PP)                                                     \
   if (T == TT && C == CC && G == GG && P == PP)                                           \
     return launch_fused_typed<TT, CC, GG, PP != 0, In>(x, stride, N, bmin, nblocks, top_key, \
-                                                       colsum, stream);
+                                                       colsum, tile_list, tile_count, stream);
   SC_FUSED_VARIANTS(SC_FUSED_CASE)
 #undef SC_FUSED_CASE
   return hipErrorInvalidValue;
@@ -1308,19 +1318,26 @@ bool fused_supported(int T, int C, int G, int P) {
 
 hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, int N, int bmin,
                                  int nblocks, int C, int G, int P, top_key_t* top_key,
-                                 float* colsum, hipStream_t stream) {
+                                 float* colsum, hipStream_t stream, const int* tile_list,
+                                 const int* tile_count) {
   switch (dtype) {
     case 0:
       return fused_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, C, G, P,
-                                   top_key, colsum, stream);
+                                   top_key, colsum, tile_list, tile_count, stream);
     case 1:
       return fused_dispatch<__half>((const __half*)x, stride, T, N, bmin, nblocks, C, G, P,
-                                    top_key, colsum, stream);
+                                    top_key, colsum, tile_list, tile_count, stream);
     case 2:
       return fused_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N, bmin,
-                                            nblocks, C, G, P, top_key, colsum, stream);
+                                            nblocks, C, G, P, top_key, colsum, tile_list,
+                                            tile_count, stream);
   }
   return hipErrorInvalidValue;
+}
+
+int fused_tile_width(int T, int C) {
+  if (T < 2 || C <= 0) return 0;
+  return (kWave - (T - 1 + C - 1) / C) * C;
 }
 
 bool two_pass_supported(int T, int G) {

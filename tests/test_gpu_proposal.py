@@ -1,0 +1,132 @@
+"""The proposal network's HIP kernels against the CPU oracle, and the gated
+search against the plain search.
+
+Bars: pooling bit-exact; first-layer activations within one BF16 ulp (the GPU
+sums in FP32 in matrix-core order, the oracle in float64, so a value near a
+rounding boundary may round the other way), deeper ones identical in >= 99 %
+of elements and within 2^-7 relative + 1 % of the layer's RMS;
+logits within 1e-2; the gradient within 3 % relative L2 error of the oracle's
+(same argmax positions); the gated search with every block flagged equals the
+plain device-statistics search hit for hit.
+"""
+
+import numpy as np
+import pytest
+
+import seticore_amd as sc
+from seticore_amd import proposal
+
+pytestmark = pytest.mark.gpu
+
+T, FOFF, TSAMP = 16, -2.79e-6, 18.25
+
+
+def _coarse_channel(torch, N, seed):
+    rng = np.random.default_rng(seed)
+    recipe = proposal.coarse_channel_recipe(rng, N, num_tones=10, num_rfi=2)
+    x = torch.empty((T, N), dtype=torch.float32, device="cuda")
+    proposal.synth(x, seed, recipe)
+    return x
+
+
+def test_pool_bit_exact(gpu, oracle):
+    import torch
+    from oracle import proposal_oracle as P
+    x = _coarse_channel(torch, 1 << 16, 1)
+    stats = torch.tensor([16.3, 4.1], dtype=torch.float32, device="cuda")
+    got = sc.ProposalNet.pool(x, stats).float().cpu().numpy()
+    want = P.pool(x.cpu().numpy(), np.float32(16.3), np.float32(4.1))
+    np.testing.assert_array_equal(got, want)
+
+
+def _tiles_and_params(torch, B, seed):
+    x = _coarse_channel(torch, B * 4096, seed)
+    col = x.sum(0)
+    stats = torch.tensor([float(col.median()), float(col.std())], dtype=torch.float32,
+                         device="cuda")
+    tiles = sc.ProposalNet.pool(x, stats)
+    net = sc.ProposalNet(max_batch=B, seed=seed)
+    p = net.get_params()
+    # Small positive biases so that every layer has active units.
+    rng = np.random.default_rng(seed)
+    from oracle import proposal_oracle as P
+    p[P.OFF_B1:P.OFF_W2] = rng.uniform(0, 0.1, P.C1)
+    p[P.OFF_B2:P.OFF_W3] = rng.uniform(0, 0.1, P.C2)
+    p[P.OFF_B3:P.OFF_WH] = rng.uniform(0, 0.1, P.C3)
+    net.set_params(p)
+    return net, tiles, p
+
+
+def _bf16_ulps(a, b):
+    ua = a.astype(np.float32).view(np.int32) >> 16
+    ub = b.astype(np.float32).view(np.int32) >> 16
+    return np.abs(ua.astype(np.int64) - ub.astype(np.int64))
+
+
+def test_forward_matches_oracle(gpu):
+    import torch
+    from oracle import proposal_oracle as P
+    B = 6
+    net, tiles, p = _tiles_and_params(torch, B, 11)
+    logits = net.forward(tiles).cpu().numpy()
+    h1, h2, h3, _, _ = net.activations(B)
+    zl, _, (r1, r2, r3) = P.forward(tiles.float().cpu().numpy(), p, bf16=True)
+    # h1 comes from identical inputs: at most one ulp apart. Deeper layers
+    # inherit those one-ulp differences of their inputs, which matter most
+    # where a sum cancels to a small value: bound them by the layer's scale.
+    g1 = h1.float().cpu().numpy()
+    assert _bf16_ulps(g1, r1).max() <= 1
+    for got, want in ((h1, r1), (h2, r2), (h3, r3)):
+        g = got.float().cpu().numpy()
+        assert (_bf16_ulps(g, want) == 0).mean() >= 0.99
+        scale = np.sqrt(np.mean(want ** 2))
+        assert np.all(np.abs(g - want) <= 2.0 ** -7 * np.abs(want) + 1e-2 * scale)
+        assert (want > 0).mean() > 0.05  # the check covers active units
+    np.testing.assert_allclose(logits, zl, rtol=1e-2, atol=1e-2)
+
+
+def test_gradient_matches_oracle(gpu):
+    import torch
+    from oracle import proposal_oracle as P
+    B = 8
+    net, tiles, p = _tiles_and_params(torch, B, 12)
+    labels = torch.tensor([1, 0, 0, 1, 0, 0, 0, 1], dtype=torch.float32, device="cuda")
+    grad, loss = net.gradient(tiles, labels, pos_weight=4.0)
+    g = grad.cpu().numpy().astype(np.float64)
+    _, _, _, zl, arg = net.activations(B)
+    ref_loss, ref = P.gradient(tiles.float().cpu().numpy(), labels.cpu().numpy(), p, 4.0)
+    _, ref_arg, _ = P.forward(tiles.float().cpu().numpy(), p, bf16=True)
+    np.testing.assert_array_equal(arg.cpu().numpy(), ref_arg)
+    assert abs(float(loss.item()) - ref_loss) <= 1e-3 * max(1.0, ref_loss)
+    err = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+    print("gradient relative L2 error %.2e" % err)
+    assert err < 3e-2
+    # Every layer carries gradient.
+    for lo, hi in [(P.OFF_W1, P.OFF_B1), (P.OFF_W2, P.OFF_B2), (P.OFF_W3, P.OFF_B3),
+                   (P.OFF_WH, P.OFF_BH)]:
+        assert np.abs(ref[lo:hi]).max() > 0
+
+
+def test_gated_search_with_every_block_flagged_equals_plain(gpu):
+    import torch
+    N = 1 << 18
+    x = _coarse_channel(torch, N, 21)
+    meta = sc.FilterbankMetadata(coarse_channel_size=N, foff=FOFF)
+    buf = sc.FilterbankBuffer(T, N, data=x)
+    dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
+    dd.set_stats_mode(sc.SC_STATS_DEVICE)
+    plain = []
+    dd.search(buf, meta, -1, 0, 10.0, 0.0, 10.0, plain)
+    st_plain = dd.state()
+    net = sc.ProposalNet(max_batch=64, seed=5)
+    gated = []
+    flagged = dd.search_proposed(buf, net, meta, -1, 0, 10.0, 0.0, 10.0, -1e30, gated)
+    assert flagged == N // 4096
+    st = dd.state()
+    np.testing.assert_array_equal(st["top_sums"], st_plain["top_sums"])
+    assert [h.key() + (h.snr,) for h in gated] == [h.key() + (h.snr,) for h in plain]
+    assert len(plain) > 0
+    # Nothing flagged: nothing searched, no hits.
+    none = []
+    assert dd.search_proposed(buf, net, meta, -1, 0, 10.0, 0.0, 10.0, 1e30, none) == 0
+    assert none == []
