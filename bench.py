@@ -107,6 +107,17 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--prefetch", type=int, default=0)
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="replay each coarse channel's search from a cached hipGraph (1) or "
+                         "launch it directly (0)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run device-to-device copies of this size (for scaling "
+                         "profiler byte counters); not timed")
+    ap.add_argument("--workload", default="plain",
+                    choices=["plain", "half", "batched", "learned"],
+                    help="plain: the file-scale search (default); half: the same with "
+                         "FP16/BF16 storage; batched: uneven beams x coarse channels in one "
+                         "graph launch; learned: proposal-gated search")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -117,12 +128,27 @@ def main():
     import torch
     import seticore_amd as sc
 
+    if args.workload != "plain":
+        {"half": run_half, "batched": run_batched, "learned": run_learned}[args.workload](
+            args, torch, sc)
+        return
+
     num_cc = args.coarse_channels
     data = generate(num_cc, torch, sc)
     per_cc, nblocks = path_sums_per_cc()
 
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     dd = sc.Dedopplerer(T, N, FOFF, TSAMP, False)
     dd.set_stats_mode(sc.SC_STATS_DEVICE)
+    dd.set_graphs(bool(args.graphs))
     if args.lanes:
         dd.set_tuning(args.lanes, args.group, args.prefetch)
     meta = sc.FilterbankMetadata(fch1=8419.0, foff=FOFF, coarse_channel_size=N)
@@ -213,6 +239,280 @@ def main():
                       "blocks), full search incl. host statistics, %d threads" % (k, threads),
         }
     print(json.dumps(line), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Further workloads (not the headline): each prints one JSON line.
+
+def _line(metric_value, unit, ms_per_step, args, dtype, data, config, **extra):
+    line = {
+        "metric": METRIC, "value": round(metric_value, 3), "unit": unit, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": data, "config": config,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+def _add_rfi_and_bursts(data, torch, seed):
+    """Stationary narrowband RFI up to 1e6 x the noise and broadband bursts on
+    3 % of the time steps at 100 x the noise, per coarse channel."""
+    rng = np.random.default_rng(seed)
+    for cc in range(data.shape[0]):
+        gain = float(data[cc, :, N // 4:N // 4 + 4096].mean())
+        for _ in range(4):
+            c = int(rng.integers(int(0.06 * N), int(0.94 * N)))
+            data[cc, :, c] += gain * float(10 ** rng.uniform(1.0, 6.0))
+        for t in range(T):
+            if rng.uniform() < 0.03:
+                data[cc, t] += 100.0 * gain
+    torch.cuda.synchronize()
+
+
+def _timed_searches(torch, fn, num_cc, steps, warmup):
+    for _ in range(warmup):
+        for cc in range(num_cc):
+            fn(cc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for cc in range(num_cc):
+            fn(cc)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_half(args, torch, sc):
+    """Reduced-precision storage: the file-scale layout plus RFI and bursts,
+    normalised per coarse channel (FP64 moments on the GPU) and stored as FP16
+    and BF16; FP32 tree accumulation. Compares throughput and hit sets with
+    the FP32 search of the raw data."""
+    num_cc = args.coarse_channels
+    data = generate(num_cc, torch, sc)
+    _add_rfi_and_bursts(data, torch, SEED + 1)
+    per_cc, nblocks = path_sums_per_cc()
+    meta = sc.FilterbankMetadata(fch1=8419.0, foff=FOFF, coarse_channel_size=N)
+    results = {}
+    hitsets = {}
+    for name, dtype in (("f32", sc.SC_F32), ("f16", sc.SC_F16), ("bf16", sc.SC_BF16)):
+        if dtype == sc.SC_F32:
+            store = data
+        else:
+            store = torch.empty((num_cc, T, N), dtype=torch.float16 if dtype == sc.SC_F16
+                                else torch.bfloat16, device="cuda")
+            for cc in range(num_cc):
+                sc.normalize(data[cc], dtype, out=store[cc])
+        dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
+        dd.set_stats_mode(sc.SC_STATS_DEVICE)
+        bufs = [sc.FilterbankBuffer(T, N, data=store[cc], dtype=dtype) for cc in range(num_cc)]
+        hits = {}
+
+        def one(cc):
+            out = []
+            dd.search(bufs[cc], meta, sc.NO_BEAM, cc, MAX_DRIFT, 0.0, SNR_THRESHOLD, out)
+            hits[cc] = out
+
+        sec = _timed_searches(torch, one, num_cc, args.steps, args.warmup)
+        results[name] = sec
+        hitsets[name] = {(cc, h.index, h.drift_steps): h.snr for cc in hits for h in hits[cc]}
+        if dtype != sc.SC_F32:
+            del store
+        del dd, bufs
+    ref = hitsets["f32"]
+    strong = {k for k, v in ref.items() if v >= 20}
+    diffs = {}
+    for name in ("f16", "bf16"):
+        got = hitsets[name]
+        common = set(ref) & set(got)
+        rel = [abs(got[k] - ref[k]) / abs(ref[k]) for k in common if abs(ref[k]) > 0]
+        diffs[name] = {
+            "hits": len(got), "only_in_f32": len(set(ref) - set(got)),
+            "only_in_half": len(set(got) - set(ref)),
+            "strong_f32_hits_found": "%d/%d" % (len(strong & set(got)), len(strong)),
+            "median_rel_snr_diff_vs_f32": float(np.median(rel)) if rel else None,
+        }
+    value = per_cc * num_cc / results["bf16"] / 1e9
+    _line(value, "Gpath-sums/s", 1000 * results["bf16"], args, "bf16 storage, f32 accumulate",
+          "synthetic (file-scale layout + narrowband RFI up to 1e6x + 3 % burst rows)",
+          {"workload": "reduced-precision storage: %d coarse channels x 2^20 x 16, normalised "
+                       "per coarse channel, max drift 10 Hz/s, SNR 10, device statistics"
+                       % num_cc,
+           "f32_gpath_per_s": round(per_cc * num_cc / results["f32"] / 1e9, 3),
+           "f16_gpath_per_s": round(per_cc * num_cc / results["f16"] / 1e9, 3),
+           "bf16_gpath_per_s": round(value, 3),
+           "f32_hits": len(ref), "vs_f32": diffs})
+
+
+def run_batched(args, torch, sc):
+    """64 beams x 32 coarse channels: time steps per beam from {16, 32, 64, 256},
+    fine channels per coarse channel from {2^17, 2^18, 2^20}, 20 tones per beam
+    at +-5 Hz/s and SNR 10-50, max drift 5 Hz/s. Each beam's 32 coarse
+    channels are generated on the GPU and searched in one graph launch
+    (autotuned shapes); the same data is searched instance by instance with
+    the plain API for comparison (hits must agree)."""
+    from seticore_amd import batch as B
+    rng = np.random.default_rng(SEED + 3)
+    beams, ccs = 64, 32
+    if args.coarse_channels != NUM_CC:
+        beams = max(1, args.coarse_channels)
+    beam_T = [int(rng.choice([16, 32, 64, 256])) for _ in range(beams)]
+    cc_N = [int(rng.choice([1 << 17, 1 << 18, 1 << 20])) for _ in range(ccs)]
+    max_drift = 5.0
+    tuning = []
+    t_tune = time.perf_counter()
+    for Tb in sorted(set(beam_T)):
+        for Nc in sorted(set(cc_N)):
+            tuning.append(B.autotune(Tb, Nc, FOFF, TSAMP, max_drift, reps=2))
+    t_tune = time.perf_counter() - t_tune
+    bufs = {Tb: [torch.empty((Tb, Nc), dtype=torch.float32, device="cuda") for Nc in cc_N]
+            for Tb in sorted(set(beam_T))}
+    batches = {}
+    for Tb, xs in bufs.items():
+        inst = [B.instance(x, Tb, FOFF, TSAMP, max_drift, 0.0, SNR_THRESHOLD, beam=-1,
+                           coarse_channel=c) for c, x in enumerate(xs)]
+        batches[Tb] = B.Batch(inst, tuning)
+    plain = {}
+    metas = {Nc: sc.FilterbankMetadata(fch1=8419.0, foff=FOFF, coarse_channel_size=Nc)
+             for Nc in set(cc_N)}
+    t_batch = t_plain = 0.0
+    total_ps = 0
+    mismatched = 0
+    total_hits = 0
+    for beam in range(beams):
+        Tb = beam_T[beam]
+        xs = bufs[Tb]
+        tones = {}
+        for _ in range(20):
+            c = int(rng.integers(0, ccs))
+            drift = float(rng.uniform(-5.0, 5.0))
+            tones.setdefault(c, []).append((float(rng.uniform(0.06, 0.94)), drift,
+                                            float(rng.uniform(10.0, 50.0))))
+        for c, x in enumerate(xs):
+            Nc = cc_N[c]
+            tl = [(f * Nc, d * TSAMP / (FOFF * 1e6), snr / math.sqrt(Tb))
+                  for f, d, snr in tones.get(c, [])]
+            sc.gen_waterfall(x, Tb, seed=SEED * 7 + beam * 1000 + c, tones=tl)
+            nb = sc.drift_parameters(Tb, FOFF, TSAMP, max_drift)
+            total_ps += (nb[1] - nb[0] + 1) * Tb * Nc
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batches[Tb].run()
+        t_batch += time.perf_counter() - t0
+        # Per-instance plain loop over the same data.
+        t0 = time.perf_counter()
+        loop_hits = []
+        for c, x in enumerate(xs):
+            Nc = cc_N[c]
+            key = (Tb, Nc)
+            if key not in plain:
+                dd = sc.Dedopplerer(Tb, Nc, FOFF, TSAMP)
+                dd.set_stats_mode(sc.SC_STATS_DEVICE)
+                plain[key] = dd
+            out = []
+            plain[key].search(sc.FilterbankBuffer(Tb, Nc, data=x), metas[Nc], sc.NO_BEAM, c,
+                              max_drift, 0.0, SNR_THRESHOLD, out)
+            loop_hits.append(out)
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        for c in range(ccs):
+            a = [h.key() + (h.snr,) for h in batches[Tb].hits(c)]
+            b = [h.key() + (h.snr,) for h in loop_hits[c]]
+            total_hits += len(b)
+            mismatched += a != b
+    value = total_ps / t_batch / 1e9
+    _line(value, "Gpath-sums/s", 1000 * t_batch, args, "f32",
+          "synthetic (seeded exponential noise + 20 tones per beam, generated on the GPU "
+          "beam by beam; generation untimed)",
+          {"workload": "batched uneven search: %d beams x %d coarse channels, time steps per "
+                       "beam from {16,32,64,256}, channels per coarse channel from "
+                       "{2^17,2^18,2^20}, max drift 5 Hz/s, SNR 10, one graph launch per beam"
+                       % (beams, ccs),
+           "path_sums": total_ps, "beam_time_steps": beam_T, "coarse_channel_widths": cc_N,
+           "plain_loop_gpath_per_s": round(total_ps / t_plain / 1e9, 3),
+           "speedup_vs_plain_loop": round(t_plain / t_batch, 3),
+           "instances_with_different_hits": mismatched, "hits": total_hits,
+           "autotune_seconds": round(t_tune, 2),
+           "tuning": [t.as_dict() for t in tuning]},
+          steps_note="one pass over all beams (the steps/warmup flags do not apply)")
+
+
+def run_learned(args, torch, sc):
+    """Proposal-gated search on the file-scale layout with the learned-proposal
+    generator (tones at SNR 8-50, drift +-10 Hz/s, narrowband RFI, bursts).
+    The network is trained here on 2^16 blocks (256 coarse channels) labelled
+    by the plain search, thresholds are fixed on 32 held-out coarse channels
+    for block recalls of 0.9, 0.99, 0.995 and 1.0, and the gated search is
+    timed against the plain search on 64 further coarse channels (FP32 and
+    BF16 storage), with hit recall versus the plain search."""
+    from seticore_amd import proposal as P
+    t0 = time.perf_counter()
+    train = P.make_dataset(256, N, seed=11)
+    val = P.make_dataset(32, N, seed=12)
+    net = sc.ProposalNet(max_batch=256, seed=13)
+    hist = P.train(net, train, epochs=30, batch=256, lr=3e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    z = P.scores(net, val.tiles)
+    thresholds = {r: P.threshold_for_recall(z, val.labels, r) for r in (0.9, 0.99, 0.995, 1.0)}
+    del train, val
+    num_cc = args.coarse_channels
+    rng = np.random.default_rng(SEED + 5)
+    recipes = [P.coarse_channel_recipe(rng, N, num_tones=4, num_rfi=1) for _ in range(num_cc)]
+    data = torch.empty((num_cc, T, N), dtype=torch.float32, device="cuda")
+    for cc in range(num_cc):
+        P.synth(data[cc], SEED * 31 + cc, recipes[cc])
+    half = torch.empty((num_cc, T, N), dtype=torch.bfloat16, device="cuda")
+    for cc in range(num_cc):
+        sc.normalize(data[cc], sc.SC_BF16, out=half[cc])
+    per_cc, nblocks = path_sums_per_cc()
+    meta = sc.FilterbankMetadata(fch1=8419.0, foff=FOFF, coarse_channel_size=N)
+    dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
+    dd.set_stats_mode(sc.SC_STATS_DEVICE)
+    dd.set_graphs(False)
+    b32 = [sc.FilterbankBuffer(T, N, data=data[cc]) for cc in range(num_cc)]
+    b16 = [sc.FilterbankBuffer(T, N, data=half[cc], dtype=sc.SC_BF16) for cc in range(num_cc)]
+    plain_hits = {}
+
+    def plain(cc):
+        out = []
+        dd.search(b32[cc], meta, sc.NO_BEAM, cc, MAX_DRIFT, 0.0, SNR_THRESHOLD, out)
+        plain_hits[cc] = {(h.index, h.drift_steps) for h in out}
+
+    t_plain = _timed_searches(torch, plain, num_cc, args.steps, args.warmup)
+    nhits = sum(len(v) for v in plain_hits.values())
+    points = []
+    for rec, thr in sorted(thresholds.items()):
+        for name, bufs in (("f32", b32), ("bf16", b16)):
+            found = [0]
+            flagged = [0]
+
+            def gated(cc):
+                out = []
+                flagged[0] += dd.search_proposed(bufs[cc], net, meta, sc.NO_BEAM, cc, MAX_DRIFT,
+                                                 0.0, SNR_THRESHOLD, thr, out)
+                if name == "f32":
+                    found[0] += len(plain_hits[cc] & {(h.index, h.drift_steps) for h in out})
+
+            _timed_searches(torch, gated, num_cc, 1, 0)  # recall and flags; warm-up
+            frac = flagged[0] / (num_cc * (N // 4096))
+            recall = found[0] / max(nhits, 1)
+            sec = _timed_searches(torch, gated, num_cc, args.steps, 0)
+            points.append({"val_block_recall": rec, "storage": name,
+                           "threshold": round(thr, 4), "flagged_fraction": round(frac, 4),
+                           "hit_recall_vs_plain": round(recall, 4) if name == "f32" else None,
+                           "equiv_gpath_per_s": round(per_cc * num_cc / sec / 1e9, 3),
+                           "speedup_vs_plain": round(t_plain / sec, 3)})
+    main_pt = [p for p in points if p["val_block_recall"] == 0.995 and p["storage"] == "f32"][0]
+    _line(main_pt["equiv_gpath_per_s"], "Gpath-sums/s (equivalent: full-search path sums / time)",
+          1000 * t_plain / main_pt["speedup_vs_plain"], args, "bf16 network, f32 search",
+          "synthetic (learned-proposal generator: tones SNR 8-50, +-10 Hz/s, RFI, bursts)",
+          {"workload": "learned proposal + full fallback: %d coarse channels x 2^20 x 16, "
+                       "4096-channel blocks, thresholds fixed on held-out data" % num_cc,
+           "plain_gpath_per_s": round(per_cc * num_cc / t_plain / 1e9, 3),
+           "plain_hits": nhits, "points": points,
+           "train_seconds_incl_data": round(t_train, 2),
+           "final_train_loss": round(hist[-1], 4)})
 
 
 if __name__ == "__main__":

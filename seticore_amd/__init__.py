@@ -531,13 +531,18 @@ def gen_waterfall(out, num_timesteps, seed, gain=1.0, taper_frac=0.0, dc_spike=0
     return out
 
 
-def normalize(data, out_dtype, clamp=65504.0, stream=None):
+def normalize(data, out_dtype, clamp=65504.0, stream=None, out=None):
     """Per-coarse-channel normalisation (x - mean) / std with FP64 moments on the
-    GPU, stored as `out_dtype`. Returns (tensor, mean, std)."""
+    GPU, stored as `out_dtype` (into `out` if given, a contiguous [T][N] CUDA
+    tensor of that type). Returns (tensor, mean, std)."""
     torch = _torch()
     T, N = data.shape
     tdtype = {SC_F32: torch.float32, SC_F16: torch.float16, SC_BF16: torch.bfloat16}[out_dtype]
-    out = torch.empty((T, N), dtype=tdtype, device=data.device)
+    if out is None:
+        out = torch.empty((T, N), dtype=tdtype, device=data.device)
+    elif out.dtype != tdtype or tuple(out.shape) != (T, N) or not out.is_contiguous():
+        raise SeticoreError(SC_ERR_INVALID_ARGUMENT, "normalize: out must be a contiguous "
+                                                     "[T][N] tensor of the output type")
     mean = ctypes.c_double()
     sd = ctypes.c_double()
     check(lib().sc_normalize(ctypes.c_void_p(data.data_ptr()), int(T), int(N),
