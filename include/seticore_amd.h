@@ -347,6 +347,46 @@ sc_status sc_batch_copy_hits(const sc_batch* b, int instance, sc_hit* hits, size
 sc_status sc_batch_info(const sc_batch* b, int* num_shapes, int* graph_nodes);
 sc_status sc_batch_destroy(sc_batch* b);
 
+/* ---- Input: sigproc filterbank files and loader conditioning ---------------
+ * Header parsing as FilReader (fil_reader.cpp:31-131), metadata inference as
+ * FilterbankMetadata::inferMetadata (filterbank_metadata.cpp:53-93). Coarse
+ * channel loading (unimplemented in the reference's FilReader,
+ * fil_reader.cpp:151-154) applies the HDF5 loader's conditioning
+ * (h5_reader.cpp:195-209): zero rows up to rounded_timesteps and, when the
+ * metadata has a DC spike, the centre channel of each row replaced by the mean
+ * of its neighbours. Errors are status codes where the reference exits. */
+
+typedef struct {
+  char source_name[256];
+  double fch1, foff, tstart, tsamp, src_raj, src_dej; /* RA hours, Dec degrees */
+  int telescope_id;                                    /* -1 if absent */
+  long num_timesteps, num_channels, coarse_channel_size, num_coarse_channels;
+  int has_dc_spike;
+} sc_fil_meta;
+
+typedef struct sc_fil sc_fil;
+
+sc_status sc_fil_open(const char* filename, sc_fil** out);
+sc_status sc_fil_metadata(const sc_fil* f, sc_fil_meta* out);
+/* Overrides the inferred coarse channel size (the reference's nfpc header). */
+sc_status sc_fil_set_coarse_channel_size(sc_fil* f, long coarse_channel_size);
+/* Coarse channel i as host float32 [rounded_timesteps][coarse_channel_size]. */
+sc_status sc_fil_read_coarse_channel(const sc_fil* f, int i, int rounded_timesteps,
+                                     float* host);
+/* The same into device memory (returns when the upload is complete). */
+sc_status sc_fil_load_coarse_channel(const sc_fil* f, int i, int rounded_timesteps,
+                                     float* device, void* stream);
+sc_status sc_fil_close(sc_fil* f);
+
+/* The loader conditioning on a coarse channel already in device memory
+ * (float32 [rounded_timesteps][row_stride]): rows num_timesteps.. are zeroed,
+ * and with has_dc_spike the centre channel (num_channels/2) of each of the
+ * first num_timesteps rows becomes the mean of its neighbours
+ * (h5_reader.cpp:195-209). */
+sc_status sc_condition_coarse_channel(float* data, int num_timesteps, int rounded_timesteps,
+                                      int num_channels, long row_stride, int has_dc_spike,
+                                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,16 @@ class ScFileMeta(ctypes.Structure):
                 ("coarse_channel_size", ctypes.c_long), ("telescope_id", ctypes.c_int)]
 
 
+class ScFilMeta(ctypes.Structure):
+    _fields_ = [("source_name", ctypes.c_char * 256), ("fch1", ctypes.c_double),
+                ("foff", ctypes.c_double), ("tstart", ctypes.c_double),
+                ("tsamp", ctypes.c_double), ("src_raj", ctypes.c_double),
+                ("src_dej", ctypes.c_double), ("telescope_id", ctypes.c_int),
+                ("num_timesteps", ctypes.c_long), ("num_channels", ctypes.c_long),
+                ("coarse_channel_size", ctypes.c_long), ("num_coarse_channels", ctypes.c_long),
+                ("has_dc_spike", ctypes.c_int)]
+
+
 _lib_handle = None
 
 
@@ -144,6 +154,13 @@ def lib():
         "sc_pn_gradient": [vp, vp, vp, i, ctypes.c_float, vp, vp, vp],
         "sc_pn_train_step": [vp, vp, vp, i, ctypes.c_float, ctypes.c_float, vp, vp],
         "sc_pn_activations": [vp, P(vp), P(vp), P(vp), P(vp), P(vp)],
+        "sc_fil_open": [ctypes.c_char_p, P(vp)],
+        "sc_fil_metadata": [vp, P(ScFilMeta)],
+        "sc_fil_set_coarse_channel_size": [vp, lng],
+        "sc_fil_read_coarse_channel": [vp, i, i, P(ctypes.c_float)],
+        "sc_fil_load_coarse_channel": [vp, i, i, vp, vp],
+        "sc_fil_close": [vp],
+        "sc_condition_coarse_channel": [vp, i, i, i, lng, i, vp],
         "sc_dd_search_proposed": [vp, vp, vp, i, i, i, lng, P(ScMeta), i, i, d, d, d,
                                   ctypes.c_float, P(ScHit), sz, P(sz), P(i), vp],
     }
@@ -664,3 +681,74 @@ def _device_view(ptr, shape, dtype):
     if hip.hipMemcpy(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ptr), n, 3) != 0:
         raise SeticoreError(SC_ERR_HIP, "hipMemcpy of activations failed")
     return out
+
+
+# ---------------------------------------------------------------------------
+# Input files (fil_reader.cpp, filterbank_metadata.cpp:53-93, h5_reader.cpp:195-209).
+
+class FilReader:
+    """Sigproc filterbank reader: header and inferred metadata on open,
+    coarse channels loaded with zero padding and the DC-spike fix."""
+
+    def __init__(self, filename):
+        h = ctypes.c_void_p()
+        check(lib().sc_fil_open(os.fsencode(filename), ctypes.byref(h)))
+        self._h = h
+        self._refresh()
+
+    def _refresh(self):
+        m = ScFilMeta()
+        check(lib().sc_fil_metadata(self._h, ctypes.byref(m)))
+        self.meta = m
+        for f, _ in ScFilMeta._fields_:
+            v = getattr(m, f)
+            setattr(self, f, v.decode("utf-8", "replace") if isinstance(v, bytes) else v)
+        self.has_dc_spike = bool(m.has_dc_spike)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().sc_fil_close(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_coarse_channel_size(self, n):
+        check(lib().sc_fil_set_coarse_channel_size(self._h, int(n)))
+        self._refresh()
+
+    def getMetadata(self):
+        return FilterbankMetadata(fch1=self.fch1, foff=self.foff,
+                                  coarse_channel_size=self.coarse_channel_size,
+                                  source_name=self.source_name, tstart=self.tstart,
+                                  tsamp=self.tsamp, src_raj=self.src_raj, src_dej=self.src_dej,
+                                  num_timesteps=self.num_timesteps,
+                                  num_channels=self.num_channels,
+                                  telescope_id=self.telescope_id,
+                                  has_dc_spike=self.has_dc_spike)
+
+    def read_coarse_channel(self, i, rounded_timesteps=None):
+        """Host numpy float32 [T_r][coarse_channel_size]."""
+        import numpy as np
+        T_r = rounded_timesteps or round_up_to_power_of_two(self.num_timesteps)
+        out = np.empty((T_r, self.coarse_channel_size), np.float32)
+        check(lib().sc_fil_read_coarse_channel(
+            self._h, int(i), int(T_r), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def loadCoarseChannel(self, i, buffer, stream=None):
+        """Into a FilterbankBuffer (device float32 [T_r][coarse_channel_size])."""
+        if buffer.num_channels != self.coarse_channel_size or buffer.row_stride != buffer.num_channels:
+            raise SeticoreError(SC_ERR_INVALID_ARGUMENT, "buffer shape mismatch")
+        check(lib().sc_fil_load_coarse_channel(self._h, int(i), buffer.num_timesteps,
+                                               ctypes.c_void_p(buffer.data.data_ptr()),
+                                               _stream_ptr(stream)))
+
+
+def condition_coarse_channel(data, num_timesteps, has_dc_spike, stream=None):
+    """Zero rows num_timesteps.. and apply the DC-spike fix in place on a CUDA
+    float32 [T_r][N] tensor (h5_reader.cpp:195-209)."""
+    T_r, N = data.shape
+    check(lib().sc_condition_coarse_channel(ctypes.c_void_p(data.data_ptr()), int(num_timesteps),
+                                            int(T_r), int(N), int(data.stride(0)),
+                                            int(bool(has_dc_spike)), _stream_ptr(stream)))
+    return data

@@ -31,6 +31,7 @@ SOURCES = [
     ("dd_capi.cpp", []),
     ("pn_capi.cpp", []),
     ("hit_writers.cpp", []),
+    ("fil_reader.cpp", []),
 ]
 
 

@@ -990,6 +990,20 @@ sc_status sc_gen_waterfall(float* out, int num_timesteps, int rounded_timesteps,
 
 }  // extern "C"
 
+extern "C" sc_status sc_condition_coarse_channel(float* data, int num_timesteps,
+                                                 int rounded_timesteps, int num_channels,
+                                                 long row_stride, int has_dc_spike,
+                                                 void* stream) {
+  if (!data) return fail(SC_ERR_INVALID_ARGUMENT, "null data");
+  if (num_timesteps < 0 || rounded_timesteps < num_timesteps || num_channels <= 0 ||
+      row_stride < num_channels)
+    return fail(SC_ERR_INVALID_ARGUMENT, "bad shape");
+  SC_HIP(sc::launch_condition(data, num_timesteps, rounded_timesteps, num_channels, row_stride,
+                              has_dc_spike, (hipStream_t)stream),
+         "condition");
+  return SC_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Batched search: many coarse channels / beams of uneven shapes in one graph.
 //

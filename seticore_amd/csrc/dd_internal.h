@@ -93,6 +93,10 @@ hipError_t launch_normalize(const float* x, int T, int N, long stride, int out_d
                             void* scratch, hipStream_t stream);
 size_t normalize_scratch_bytes(int T, int N);
 
+// Loader conditioning (zero rows T..T_r-1; optional DC fix on rows < T).
+hipError_t launch_condition(float* data, int T, int T_r, int N, long stride, int dc_spike,
+                            hipStream_t stream);
+
 hipError_t launch_gen_waterfall(float* out, int T, int T_r, int N, uint64_t seed, float gain,
                                 float taper_frac, float dc_spike, int num_tones,
                                 const float* tone_start, const float* tone_slope,

@@ -1132,6 +1132,23 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+// h5_reader.cpp:195-200: rows beyond the file's time steps are zero.
+__global__ void zero_rows_kernel(float* __restrict__ rows, int nrows, int N, long stride) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)nrows * N;
+       i += (long)gridDim.x * blockDim.x)
+    rows[(i / N) * stride + i % N] = 0.0f;
+}
+
+// h5_reader.cpp:202-209: the centre channel of each row becomes the mean of its
+// neighbours ((a + b) / 2.0 in the reference; halving is exact, so * 0.5f).
+__global__ void dc_fix_kernel(float* __restrict__ data, int T, int N, long stride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  float* row = data + (long)t * stride;
+  const int mid = N / 2;
+  row[mid] = (row[mid - 1] + row[mid + 1]) * 0.5f;
+}
+
 __global__ void gen_noise_kernel(float* __restrict__ out, int T, int T_r, int N, uint64_t seed,
                                  float gain, float taper_frac, float dc_spike) {
   const long total = (long)T_r * N;
@@ -1495,6 +1512,17 @@ hipError_t launch_normalize(const float* x, int T, int N, long stride, int out_d
       break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_condition(float* data, int T, int T_r, int N, long stride, int dc_spike,
+                            hipStream_t stream) {
+  if (T_r > T)
+    hipLaunchKernelGGL(zero_rows_kernel, dim3(blocks_for((long)(T_r - T) * N, 256, 8192)),
+                       dim3(256), 0, stream, data + (long)T * stride, T_r - T, N, stride);
+  if (dc_spike && N >= 3 && T > 0)
+    hipLaunchKernelGGL(dc_fix_kernel, dim3((T + 255) / 256), dim3(256), 0, stream, data, T, N,
+                       stride);
   return hipGetLastError();
 }
 

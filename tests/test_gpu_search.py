@@ -343,3 +343,33 @@ def test_cpp_facade_known_answer(gpu, tmp_path):
     assert dat[-1].split("\t")[5] == "70"
     hits = read_hits(str(tmp_path / "kat.hits"))
     assert len(hits) == 1 and hits[0]["signal"]["drift_steps"] == 3
+
+
+# --- loader conditioning (h5_reader.cpp:195-209) and .fil input -------------
+def test_fil_load_and_condition_on_device(gpu, oracle, tmp_path):
+    import torch
+    from test_fil_reader import write_fil
+    T, N = 15, 1 << 20
+    rng = np.random.default_rng(9)
+    data = rng.exponential(1.0, size=(T, N)).astype(np.float32)
+    data[:, N // 2] = 50.0
+    for t in range(T):
+        data[t, 300000 + 2 * t] += 4.0
+    p = tmp_path / "c.fil"
+    write_fil(p, data)
+    r = sc.FilReader(str(p))
+    buf = sc.FilterbankBuffer(16, N)
+    r.loadCoarseChannel(0, buf)
+    host = r.read_coarse_channel(0)
+    np.testing.assert_array_equal(buf.data.cpu().numpy(), host)
+    # The same conditioning on data already on the device.
+    raw = torch.from_numpy(np.vstack([data, np.full((1, N), 7.0, np.float32)])).cuda()
+    sc.condition_coarse_channel(raw, T, True)
+    np.testing.assert_array_equal(raw.cpu().numpy(), host)
+    # Search of the loaded channel equals the oracle on the conditioned data.
+    dd = sc.Dedopplerer(T, N, r.foff, r.tsamp)
+    hits = []
+    dd.search(buf, r.getMetadata(), sc.NO_BEAM, 0, 1.0, 0.0, 10.0, hits)
+    ref = oracle.search(host, T, r.foff, r.tsamp, 1.0, 0.0, 10.0, fch1=r.fch1, num_threads=8)
+    assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
+    assert all(h.index != N // 2 for h in hits)  # the DC spike is gone
