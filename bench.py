@@ -97,6 +97,7 @@ def load_traffic():
 
 
 def main():
+    global T, MAX_DRIFT
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -113,12 +114,20 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run device-to-device copies of this size (for scaling "
                          "profiler byte counters); not timed")
+    ap.add_argument("--time-steps", type=int, default=T,
+                    help="time steps per coarse channel for the plain workload (default 16)")
+    ap.add_argument("--max-drift", type=float, default=MAX_DRIFT,
+                    help="max drift in Hz/s for the plain workload (default 10)")
     ap.add_argument("--workload", default="plain",
                     choices=["plain", "half", "batched", "learned"],
                     help="plain: the file-scale search (default); half: the same with "
                          "FP16/BF16 storage; batched: uneven beams x coarse channels in one "
                          "graph launch; learned: proposal-gated search")
     args = ap.parse_args()
+    T, MAX_DRIFT = args.time_steps, args.max_drift
+    if T not in (2, 4, 8, 16, 32, 64, 128, 256, 512, 1024):
+        print("--time-steps must be a power of two from 2 to 1024", file=sys.stderr)
+        sys.exit(2)
 
     if args.gpus != 1:
         print("bench.py measures one GPU: the search shards by coarse channel with no "
@@ -203,8 +212,8 @@ def main():
         "data": "synthetic (seeded exponential noise + 200 tones, generated on the GPU)",
         "config": {
             "workload": "plain search, file scale: %d coarse channels x 2^20 fine channels x "
-                        "16 time steps float32, max drift 10 Hz/s (132 drift blocks), "
-                        "SNR 10, device statistics" % num_cc,
+                        "%d time steps float32, max drift %g Hz/s (%d drift blocks), "
+                        "SNR 10, device statistics" % (num_cc, T, MAX_DRIFT, nblocks),
             "coarse_channels": num_cc, "fine_channels": N, "time_steps": T,
             "drift_blocks": nblocks, "path_sums_per_step": total_ps,
             "hits_per_step": len(hits),
@@ -215,8 +224,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(),
-            "kernel": "fused_tree_top_kernel",
+            "traffic": load_traffic() if (T, MAX_DRIFT) == (16, 10.0) else None,
+            "kernel": "fused_tree_top_kernel" if T <= 16 else
+                      "column sums + two-pass tree (level16_kernel, upper_tree_kernel)",
             "kernel_ms_per_launch": round(avg_ms, 4),
             "alg_bytes_per_launch": alg_bytes,
         },
@@ -235,8 +245,9 @@ def main():
             "unit": "Gpath-sums/s",
             "cores": threads,
             "kind": "port",
-            "sample": "%d coarse channel(s) of the same workload (16 x 2^20, 132 drift "
-                      "blocks), full search incl. host statistics, %d threads" % (k, threads),
+            "sample": "%d coarse channel(s) of the same workload (%d x 2^20, %d drift "
+                      "blocks), full search incl. host statistics, %d threads"
+                      % (k, T, nblocks, threads),
         }
     print(json.dumps(line), flush=True)
 

@@ -90,8 +90,10 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
 
 /* Shape of the fused tree kernel used when rounded_timesteps <= 16:
  * channels held per lane (4 or 8), drift blocks per LDS window (a multiple
- * of channels_per_lane) and whether the next window is prefetched into
- * registers (0/1). Returns SC_ERR_UNSUPPORTED for shapes that are not
+ * of channels_per_lane) and the window mode in `prefetch` (0: one LDS window
+ * per workgroup; 1: each workgroup walks several windows, prefetching the
+ * next into registers; 2: several windows, each loaded after the previous
+ * one is done). Returns SC_ERR_UNSUPPORTED for shapes that are not
  * compiled. Results do not depend on the shape; only speed does. No
  * counterpart in the reference (its tile shape is fixed, taylor.cu:170-189);
  * used by the autotuner. */
@@ -100,7 +102,7 @@ sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, in
 /* Shape of the two-pass search used when rounded_timesteps is 32..256 (the
  * reference's tiledTaylorTree<32> / twoStageTaylorTree range, taylor.cu:261-350):
  * drift blocks per LDS window in the level-16 pass (4 or 8) and drift blocks
- * per launch pair (1..64; 0 = as many as keep the scratch within 256 MiB, at
+ * per launch pair (1..64; 0 = as many as keep the scratch within 4.5 GB, at
  * most drift_group). May grow the scratch (and drops cached graphs when it
  * does). Results do not depend on the shape. No counterpart in the reference;
  * used by the autotuner. */

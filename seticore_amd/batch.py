@@ -112,8 +112,8 @@ class Batch:
 # ---------------------------------------------------------------------------
 # Autotuner.
 
-FUSED_CANDIDATES = [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1), (8, 8, 0),
-                    (8, 16, 0), (8, 4, 0), (4, 16, 1)]
+FUSED_CANDIDATES = [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1), (4, 4, 2),
+                    (4, 8, 2), (4, 16, 2), (8, 8, 0), (8, 16, 0), (8, 4, 0)]
 TWO_PASS_CANDIDATES = [(4, 1), (4, 2), (4, 4), (8, 4), (8, 8)]
 
 _cache = {}

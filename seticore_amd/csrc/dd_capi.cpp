@@ -60,9 +60,11 @@ size_t dtype_size(int dtype) { return dtype == SC_F32 ? 4 : 2; }
 // Default fused-kernel shape per T_r: channels per lane and drift blocks per
 // LDS window.
 void default_tuning(int T, int* lanes, int* group, int* prefetch) {
+  // Measured on MI355X at 16 x 2^20, 132 drift blocks: C=4, G=4 with chunked
+  // windows (mode 2) is the fastest T=16 shape.
   *lanes = (T == 16) ? 4 : 8;
-  *group = (T == 16) ? 8 : 16;
-  *prefetch = 0;
+  *group = (T == 16) ? 4 : 16;
+  *prefetch = (T == 16) ? 2 : 0;
 }
 
 }  // namespace
@@ -225,11 +227,12 @@ sc_status ensure_tree_scratch(sc_dd* dd) {
 }
 
 // Drift blocks per launch pair of the two-pass search: as set, or as many as
-// keep the scratch within 256 MiB (at least one, at most the pass-1 group).
+// keep the scratch within 4.5 GB (at least one, at most the pass-1 group;
+// more blocks per launch share each input window across blocks).
 int pass_blocks(const sc_dd* dd) {
   if (dd->pass_blocks > 0) return dd->pass_blocks;
   const double per_block = 4.0 * dd->rounded * (double)dd->pass_pitch;
-  int nb = (int)((256.0 * (1 << 20)) / per_block);
+  int nb = (int)(4.5e9 / per_block);
   return std::max(1, std::min(dd->pass_group, nb));
 }
 

@@ -291,13 +291,16 @@ __device__ __forceinline__ float ord_val(unsigned k) {
 
 // Grid: one 64-lane workgroup per work item = (channel tile, chunk of
 // consecutive drift groups of G blocks). The workgroup walks its chunk window
-// by window and keeps the running key max in registers across the chunk; with
-// PF it fetches the next window's rows into registers while it runs the
-// current window's blocks out of LDS (without PF a chunk is one window). Items are tile-major and dealt so that each XCD
+// by window and keeps the running key max in registers across the chunk.
+// MODE 0: a chunk is one window. MODE 1: the next window's rows are fetched
+// into registers while the current window's blocks run out of LDS. MODE 2:
+// the next window is fetched after the current one is done (no registers held
+// across the compute); chunks cut the number of atomic merges per channel.
+// Items are tile-major and dealt so that each XCD
 // (workgroups go round-robin to the 8 XCDs) sweeps a contiguous range of
 // tiles. The workgroup whose chunk holds drift block 0 also writes the tile's
 // column sums in the reference's sequential order (dedoppler.cu:68-71).
-template <int T, int C, int G, bool PF, typename In>
+template <int T, int C, int G, int MODE, typename In>
 __global__ void __launch_bounds__(kWave)
     fused_tree_top_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
                           int ngroups, int chunks, int groups_per_chunk, int total_items,
@@ -347,10 +350,11 @@ __global__ void __launch_bounds__(kWave)
   for (int j = 0; j < C; ++j) best[j] = __longlong_as_double((long long)kEmptyKey);
 
   for (int grp = g_begin; grp < g_end; ++grp) {
-    // Without PF every work item is one drift group (the launcher makes
+    // MODE 0: every work item is one drift group (the launcher makes
     // groups_per_chunk 1), so there is no next window to fetch.
-    const bool more = PF && grp + 1 < g_end;
-    if (more) fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + (grp + 1) * G, 0);
+    const bool more = MODE != 0 && grp + 1 < g_end;
+    if (MODE == 1 && more)
+      fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + (grp + 1) * G, 0);
     const int b0 = bmin + grp * G;
     const int ng = min(G, nblocks - grp * G);
     const unsigned low0 = (unsigned)kCodeMask - (unsigned)(grp * G * T);
@@ -362,6 +366,7 @@ __global__ void __launch_bounds__(kWave)
       for (int t = 1; t < T; ++t) rp[t] += t;  // next h step: rows shift by t columns
     }
     if (more) {
+      if (MODE == 2) fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + (grp + 1) * G, 0);
       __syncthreads();
       store_window<T, C, G>(win, pf);
       __syncthreads();
@@ -1194,26 +1199,27 @@ inline int blocks_for(long n, int threads, int cap) {
   return (int)b;
 }
 
-template <int T, int C, int G, bool PF, typename In>
+template <int T, int C, int G, int MODE, typename In>
 hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nblocks,
                               top_key_t* top_key, float* colsum, const int* tile_list,
                               const int* tile_count, hipStream_t stream) {
   using Sh = TileShape<T, C>;
   const long ntiles = ((long)N + Sh::kOut - 1) / Sh::kOut;
   const long ngroups = ((long)nblocks + G - 1) / G;
-  // With PF: chunks of drift groups per tile, as long as the chip is still
+  // MODE 1/2: chunks of drift groups per tile, as long as the chip is still
   // filled several times over (256 CUs x a few resident workgroups).
   long chunks = ngroups;
-  if (PF) {
+  if (MODE != 0) {
+    const long target = MODE == 1 ? 8192 : 16384;
     chunks = 1;
-    while (ntiles * chunks < 8192 && chunks < ngroups) ++chunks;
+    while (ntiles * chunks < target && chunks < ngroups) ++chunks;
   }
   const long per_chunk = (ngroups + chunks - 1) / chunks;
   chunks = (ngroups + per_chunk - 1) / per_chunk;
   const long total = ntiles * chunks;
   if (total > (1L << 30)) return hipErrorInvalidValue;
   const int grid = (int)((total + 7) / 8 * 8);
-  hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, PF, In>), dim3(grid), dim3(kWave), 0,
+  hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, MODE, In>), dim3(grid), dim3(kWave), 0,
                      stream, x, stride, N, bmin, nblocks, (int)ngroups, (int)chunks,
                      (int)per_chunk, (int)total, top_key, colsum, tile_list, tile_count);
   return hipGetLastError();
@@ -1231,6 +1237,9 @@ hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nbl
   X(16, 4, 16, 0)            \
   X(16, 4, 4, 1)             \
   X(16, 4, 8, 1)             \
+  X(16, 4, 4, 2)             \
+  X(16, 4, 8, 2)             \
+  X(16, 4, 16, 2)            \
   X(16, 8, 8, 0)
 
 template <typename In>
@@ -1239,7 +1248,7 @@ hipError_t fused_dispatch(const In* x, long stride, int T, int N, int bmin, int 
                           const int* tile_count, hipStream_t stream) {
 #define SC_FUSED_CASE(TT, CC, GG, PP)                                                     \
   if (T == TT && C == CC && G == GG && P == PP)                                           \
-    return launch_fused_typed<TT, CC, GG, PP != 0, In>(x, stride, N, bmin, nblocks, top_key, \
+    return launch_fused_typed<TT, CC, GG, PP, In>(x, stride, N, bmin, nblocks, top_key,      \
                                                        colsum, tile_list, tile_count, stream);
   SC_FUSED_VARIANTS(SC_FUSED_CASE)
 #undef SC_FUSED_CASE

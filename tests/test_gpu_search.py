@@ -149,7 +149,8 @@ def test_search_parity_host_exact(gpu, oracle, case):
         assert _hit_fields(a) == _hit_fields(b)
 
 
-@pytest.mark.parametrize("tuning", [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1), (8, 8, 0)])
+@pytest.mark.parametrize("tuning", [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1),
+                                    (4, 4, 2), (4, 8, 2), (4, 16, 2), (8, 8, 0)])
 def test_tunings_are_bit_identical(gpu, oracle, tuning):
     T, N = 16, 50000
     data = _waterfall(T, N, 21, [(1000.5, 1.3, 3.0), (30000.0, -2.2, 2.0)])
