@@ -88,7 +88,10 @@ sc_status sc_dd_memory_usage(const sc_dd* dd, size_t* bytes);
 /* Selects sc_stats_mode for later searches (no counterpart; default host-exact). */
 sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
 
-/* Shape of the fused tree kernel used when rounded_timesteps <= 16:
+/* Shape of the one-pass tree kernel used when rounded_timesteps <= 32
+ * (at 32 `prefetch` is the number of waves per workgroup sharing the window:
+ * (C, G, waves) in {(4,4,0), (2,2,0), (2,4,0), (2,4,2), (2,8,2), (2,8,4),
+ * (4,8,2)}, 0 meaning one; selecting it undoes sc_dd_set_two_pass):
  * channels held per lane (4 or 8), drift blocks per LDS window (a multiple
  * of channels_per_lane) and the window mode in `prefetch` (0: one LDS window
  * per workgroup; 1: each workgroup walks several windows, prefetching the
@@ -99,7 +102,8 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
  * used by the autotuner. */
 sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, int prefetch);
 
-/* Shape of the two-pass search used when rounded_timesteps is 32..256 (the
+/* Shape of the two-pass search used when rounded_timesteps is 64..256, and at
+ * 32 once this is called (the default there is the one-pass kernel) (the
  * reference's tiledTaylorTree<32> / twoStageTaylorTree range, taylor.cu:261-350):
  * drift blocks per LDS window in the level-16 pass (4 or 8) and drift blocks
  * per launch pair (1..64; 0 = as many as keep the scratch within 4.5 GB, at
@@ -324,9 +328,10 @@ typedef struct {
   double max_drift, min_drift, snr_threshold;
 } sc_instance;
 
-/* Kernel shape for one (rounded_timesteps, num_channels): the fused-kernel
- * fields apply when rounded_timesteps <= 16, the two-pass fields for 32..256
- * (see sc_dd_set_tuning / sc_dd_set_two_pass). Produced by the autotuner. */
+/* Kernel shape for one (rounded_timesteps, num_channels): the one-pass fields
+ * apply when rounded_timesteps <= 16, or at 32 when pass_group is 0; the
+ * two-pass fields otherwise, for 32..256 (see sc_dd_set_tuning /
+ * sc_dd_set_two_pass). Produced by the autotuner. */
 typedef struct {
   int rounded_timesteps;
   int num_channels;

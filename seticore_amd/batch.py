@@ -114,6 +114,7 @@ class Batch:
 
 FUSED_CANDIDATES = [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1), (4, 4, 2),
                     (4, 8, 2), (4, 16, 2), (8, 8, 0), (8, 16, 0), (8, 4, 0)]
+FUSED32_CANDIDATES = [(4, 4, 0), (2, 2, 0), (2, 4, 0), (2, 4, 2), (2, 8, 2), (2, 8, 4), (4, 8, 2)]
 TWO_PASS_CANDIDATES = [(4, 1), (4, 2), (4, 4), (8, 4), (8, 8)]
 
 _cache = {}
@@ -153,8 +154,8 @@ def autotune(T_r, N, foff, tsamp, max_drift, reps=3, cache_file=None, log=None):
     dd.set_stats_mode(SC_STATS_DEVICE)
     dd.set_graphs(False)
     best = None
-    if T_r <= 16:
-        for c, g, p in FUSED_CANDIDATES:
+    if T_r <= 32:
+        for c, g, p in (FUSED_CANDIDATES if T_r <= 16 else FUSED32_CANDIDATES):
             try:
                 dd.set_tuning(c, g, p)
             except SeticoreError:
@@ -163,8 +164,8 @@ def autotune(T_r, N, foff, tsamp, max_drift, reps=3, cache_file=None, log=None):
             if log:
                 log("T=%d N=%d fused C=%d G=%d P=%d: %.3f ms" % (T_r, N, c, g, p, ms))
             if best is None or ms < best[0]:
-                best = (ms, ScTuning(T_r, N, c, g, p, 4, 0))
-    else:
+                best = (ms, ScTuning(T_r, N, c, g, p, 4 if T_r <= 16 else 0, 0))
+    if T_r >= 32:
         for g, nb in TWO_PASS_CANDIDATES:
             try:
                 dd.set_two_pass(g, nb)

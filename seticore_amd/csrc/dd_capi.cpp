@@ -60,11 +60,13 @@ size_t dtype_size(int dtype) { return dtype == SC_F32 ? 4 : 2; }
 // Default fused-kernel shape per T_r: channels per lane and drift blocks per
 // LDS window.
 void default_tuning(int T, int* lanes, int* group, int* prefetch) {
-  // Measured on MI355X at 16 x 2^20, 132 drift blocks: C=4, G=4 with chunked
-  // windows (mode 2) is the fastest T=16 shape.
-  *lanes = (T == 16) ? 4 : 8;
-  *group = (T == 16) ? 4 : 16;
-  *prefetch = (T == 16) ? 2 : 0;
+  // Measured on MI355X: at 16 x 2^20 (132 drift blocks) C=4, G=4 with chunked
+  // windows (mode 2) is the fastest T=16 shape; at 32 x 2^20 (66 blocks) the
+  // one-pass kernel with C=2, G=4 and two waves per workgroup (2.26 ms, vs
+  // 4.25 ms for the two-pass search).
+  *lanes = T == 16 ? 4 : T == 32 ? 2 : 8;
+  *group = (T == 16 || T == 32) ? 4 : 16;
+  *prefetch = (T == 16 || T == 32) ? 2 : 0;
 }
 
 }  // namespace
@@ -103,6 +105,7 @@ struct sc_dd {
   int pass_group = 4;
   int pass_blocks = 0;  // drift blocks per launch pair; 0 = from the size
   int pass_alloc_blocks = 0;
+  bool use_two_pass = false;  // T_r = 32: two passes instead of the one-pass kernel
   // Gated search (learned block proposal): channel-tile list and counters.
   int* gate_list = nullptr;    // one entry per channel tile
   int* gate_count = nullptr;   // {tile count, flagged blocks}
@@ -130,6 +133,7 @@ struct sc_dd {
     double snr;
     int lanes, group, prefetch;
     int pass_group, pass_blocks;
+    bool use_two_pass;
     hipGraphExec_t exec;
   };
   bool graphs = true;
@@ -425,7 +429,9 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
   if (s == SC_OK)
     s = host_alloc((void**)&dd->h_cand, dd->cand_capacity * sizeof(int4), "candidates");
   if (s == SC_OK) s = host_alloc((void**)&dd->h_stats, 4 * sizeof(float), "stats");
-  if (sc::two_pass_supported(rounded, dd->pass_group)) {
+  if (rounded == 32) {
+    // The one-pass kernel needs no scratch; sc_dd_set_two_pass allocates it.
+  } else if (sc::two_pass_supported(rounded, dd->pass_group)) {
     dd->pass_pitch = sc::two_pass_row_pitch(rounded, num_channels);
     if (s == SC_OK) s = ensure_pass_scratch(dd, pass_blocks(dd));
   } else if (s == SC_OK && rounded > 16) {
@@ -471,12 +477,13 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode) {
 
 sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, int prefetch) {
   if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
-  if (dd->rounded > 16) return fail(SC_ERR_UNSUPPORTED, "no fused kernel above 16 time steps");
+  if (dd->rounded > 32) return fail(SC_ERR_UNSUPPORTED, "no one-pass kernel above 32 time steps");
   if (!sc::fused_supported(dd->rounded, channels_per_lane, drift_group, prefetch))
     return fail(SC_ERR_UNSUPPORTED, "no fused kernel for this (lanes, group, prefetch) shape");
   dd->lanes = channels_per_lane;
   dd->group = drift_group;
   dd->prefetch = prefetch;
+  dd->use_two_pass = false;
   return SC_OK;
 }
 
@@ -488,6 +495,8 @@ sc_status sc_dd_set_two_pass(sc_dd* dd, int drift_group, int blocks_per_launch) 
     return fail(SC_ERR_INVALID_ARGUMENT, "blocks_per_launch must be in [0, 64]");
   dd->pass_group = drift_group;
   dd->pass_blocks = blocks_per_launch;
+  dd->use_two_pass = true;
+  if (!dd->pass_pitch) dd->pass_pitch = sc::two_pass_row_pitch(dd->rounded, dd->num_channels);
   const int nb = pass_blocks(dd);
   if (nb > dd->pass_alloc_blocks) {
     // Cached graphs hold the old scratch address.
@@ -512,7 +521,8 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
   // positive path"), plus the statistics histograms.
   SC_HIP(sc::launch_reset(dd->top_key, N, dd->stats_hist, stream), "reset");
   if (dd->timing) SC_HIP(hipEventRecord(dd->ev_start, stream), "timing event");
-  if (T <= 16 && sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch)) {
+  if ((T <= 16 || (T == 32 && !dd->use_two_pass)) &&
+      sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch)) {
     SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
                                      dd->group, dd->prefetch, dd->top_key, dd->colsum, stream),
            "fused tree");
@@ -570,7 +580,8 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
     if (g.data == data && g.dtype == dtype && g.stride == row_stride && g.bmin == r.bmin &&
         g.nblocks == nblocks && g.window == r.window && g.snr == snr_threshold &&
         g.lanes == dd->lanes && g.group == dd->group && g.prefetch == dd->prefetch &&
-        g.pass_group == dd->pass_group && g.pass_blocks == pass_blocks(dd)) {
+        g.pass_group == dd->pass_group && g.pass_blocks == pass_blocks(dd) &&
+        g.use_two_pass == dd->use_two_pass) {
       *out = g.exec;
       return SC_OK;
     }
@@ -598,7 +609,7 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
   }
   dd->graph_cache.push_back({data, dtype, row_stride, r.bmin, nblocks, r.window, snr_threshold,
                              dd->lanes, dd->group, dd->prefetch, dd->pass_group,
-                             pass_blocks(dd), exec});
+                             pass_blocks(dd), dd->use_two_pass, exec});
   *out = exec;
   return SC_OK;
 }
@@ -1087,7 +1098,7 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
       for (int k = 0; k < num_tuning; ++k) {
         const sc_tuning& t = tuning[k];
         if (t.rounded_timesteps != T || t.num_channels != in.num_channels) continue;
-        if (T <= 16)
+        if (T <= 16 || (T == 32 && t.pass_group == 0))
           s = sc_dd_set_tuning(dd, t.channels_per_lane, t.drift_group, t.prefetch);
         else if (sc::two_pass_supported(T, t.pass_group))
           s = sc_dd_set_two_pass(dd, t.pass_group, t.pass_blocks);

@@ -417,6 +417,169 @@ __global__ void __launch_bounds__(kWave)
 }
 
 // ---------------------------------------------------------------------------
+// T_r = 32 in one pass: both 16-row time blocks' trees in registers, the top
+// level through LDS.
+//
+// In the sheared coordinates of the whole 32-row window (row t at c + t*b),
+// level 32 combines offset q of the first block's tree at channel c with
+// offset q of the second block's tree at channel c + q + r (p = 2q + r): the
+// reference's upper-half shift (p+1)/2 + b*16 (taylor.cu:97-109 with
+// taylor.h:53-54), with b*16 absorbed by the shear. The second tree is
+// computed first and parked in LDS, de-interleaved by C like the window, so
+// each lane reads its shifted neighbours conflict-free; the first tree stays
+// in registers. Outputs are exact for lanes whose shifted reads stay inside
+// the second tree's exact lanes (56 of 64 at C = 4: 224 channels per wave).
+
+template <int C>
+struct Tile32 {
+  static constexpr int kValid16 = TileShape<16, C>::kValidLanes;
+  static constexpr int kValidLanes = kValid16 - 1 - 15 / C;
+  static constexpr int kOut = kValidLanes * C;
+  static constexpr int kHS = kWave + (C + 16 + C - 1) / C;  // LDS row length of the parked tree
+};
+
+// Rows R0 .. R0+NR-1 of drift block gp of the current h step.
+template <int T, int C, int G, int gp, int R0, int NR>
+__device__ __forceinline__ void read_rows(lds_float* const (&rp)[T], float (&v)[NR][C]) {
+  using Wn = Window<T, C, G>;
+#pragma unroll
+  for (int t = 0; t < NR; ++t)
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int k = j + (R0 + t) * gp;
+      v[t][j] = rp[R0 + t][(k % C) * Wn::S + k / C];
+    }
+}
+
+// Orders this wave's LDS accesses (the parked tree is private to the wave):
+// the workgroup-scope fences wait for the wave's outstanding LDS operations.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <int C, int G, int gp>
+__device__ __forceinline__ void fused32_block(lds_float* const (&rp)[32], lds_float* hp, int g,
+                                              int ng, int b0, int c_w, int N, unsigned low0,
+                                              double (&best)[C]) {
+  if constexpr (gp < C) {
+    if (g + gp < ng) {
+      using Tl = Tile32<C>;
+      const int b = b0 + g + gp;
+      float v[16][C];
+      read_rows<32, C, G, gp, 16, 16>(rp, v);
+      tree_regs<16, C>(v);
+      wave_lds_sync();  // the previous block's reads of the parked tree are done
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+#pragma unroll
+        for (int j = 0; j < C; ++j) hp[(q * C + j) * Tl::kHS] = v[q][j];
+      read_rows<32, C, G, gp, 0, 16>(rp, v);
+      tree_regs<16, C>(v);
+      wave_lds_sync();
+      const int lane = threadIdx.x % kWave;
+      const bool interior = (c_w + 31 * b >= 0) && (c_w + Tl::kOut - 1 < N) &&
+                            (c_w + Tl::kOut - 1 + 31 * b + 31 < N);
+      const unsigned lowg = low0 - (unsigned)((g + gp) * 32);
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          float sum[C];
+#pragma unroll
+          for (int j = 0; j < C; ++j) {
+            const int k = j + q + r;
+            sum[j] = v[q][j] + hp[(q * C + k % C) * Tl::kHS + k / C];
+          }
+          if (!interior) {
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+              const int c = c_w + lane * C + j;
+              const int last0 = c + 31 * b;
+              const int pmax = (c >= N || last0 < 0) ? 0 : min(32, N - last0);
+              if (2 * q + r >= pmax) sum[j] = 0.0f;
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < C; ++j)
+            best[j] = fmax(best[j], make_key(sum[j], lowg - (unsigned)(2 * q + r)));
+        }
+    }
+    fused32_block<C, G, gp + 1>(rp, hp, g, ng, b0, c_w, N, low0, best);
+  }
+}
+
+// One workgroup of W waves per (channel tile of Tile32<C>::kOut channels, group
+// of G drift blocks): the waves share the group's LDS window and take turns
+// over its drift blocks (wave w: h steps w, w + W, ...), each with its own
+// parked tree; their key maxima merge through the atomics. The workgroup
+// whose group holds block 0 writes the column sums (32 rows, sequential
+// order, dedoppler.cu:68-71).
+template <int C, int G, int W, typename In>
+__global__ void __launch_bounds__(kWave * W)
+    fused32_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
+                   int ngroups, int total_items, unsigned long long* __restrict__ top_key,
+                   float* __restrict__ colsum) {
+  static_assert(G % (C * W) == 0, "each wave takes whole h steps");
+  using Tl = Tile32<C>;
+  using Wn = Window<32, C, G>;
+  __shared__ float win_storage[Wn::kFloats];
+  __shared__ float park[W][16 * C * Tl::kHS];
+  lds_float* win = (lds_float*)win_storage;
+  const int per_xcd = gridDim.x / 8;
+  const int item = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (item >= total_items) return;
+  const int tile = item / ngroups;
+  const int grp = item - tile * ngroups;
+  const int c_w = tile * Tl::kOut;
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  const int b0 = bmin + grp * G;
+  const int ng = min(G, nblocks - grp * G);
+  if (wave == 0) load_window<32, C, G, In>(win, x, stride, N, c_w, b0, 0);
+  if (wave == 0 && colsum && b0 <= 0 && 0 < b0 + ng && lane < Tl::kValidLanes) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int c = c_w + lane * C + j;
+      if (c < N) {
+        float s = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 32; ++t) s += to_f32<In>(x[(long)t * stride + c]);
+        colsum[c] = s;
+      }
+    }
+  }
+  __syncthreads();
+  lds_float* rp[32];
+#pragma unroll
+  for (int t = 0; t < 32; ++t) {
+    lds_float* p = win + t * Wn::kRow + lane + t * wave;  // this wave's first h step
+    asm volatile("" : "+v"(p));
+    rp[t] = p;
+  }
+  lds_float* hp = (lds_float*)park[wave] + lane;
+  asm volatile("" : "+v"(hp));
+  double best[C];
+#pragma unroll
+  for (int j = 0; j < C; ++j) best[j] = __longlong_as_double((long long)kEmptyKey);
+  const unsigned low0 = (unsigned)kCodeMask - (unsigned)(grp * G * 32);
+  for (int g = C * wave; g < ng; g += C * W) {
+    fused32_block<C, G, 0>(rp, hp, g, ng, b0, c_w, N, low0, best);
+#pragma unroll
+    for (int t = 1; t < 32; ++t) rp[t] += t * W;
+  }
+  if (lane < Tl::kValidLanes) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int c = c_w + lane * C + j;
+      const unsigned long long k = (unsigned long long)__double_as_longlong(best[j]);
+      if (c < N && k > kEmptyKey) atomicMax(&top_key[c], k);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // T_r = 16*M (M = 2..16): two passes through a scratch array.
 //
 // Unrolling the recursion (taylor.h:53-54,80-82) from level T down to level
@@ -1255,6 +1418,39 @@ hipError_t fused_dispatch(const In* x, long stride, int T, int N, int bmin, int 
   return hipErrorInvalidValue;
 }
 
+template <int C, int G, int W, typename In>
+hipError_t launch_fused32_typed(const In* x, long stride, int N, int bmin, int nblocks,
+                                top_key_t* top_key, float* colsum, hipStream_t stream) {
+  using Tl = Tile32<C>;
+  const long ntiles = ((long)N + Tl::kOut - 1) / Tl::kOut;
+  const long ngroups = ((long)nblocks + G - 1) / G;
+  const long total = ntiles * ngroups;
+  if (total > (1L << 30)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((fused32_kernel<C, G, W, In>), dim3((int)((total + 7) / 8 * 8)),
+                     dim3(kWave * W), 0, stream, x, stride, N, bmin, nblocks, (int)ngroups,
+                     (int)total, top_key, colsum);
+  return hipGetLastError();
+}
+
+// T=32 shapes (C, G, waves per workgroup): `prefetch` selects the waves (0: one).
+template <typename In>
+hipError_t fused32_dispatch(const In* x, long stride, int N, int bmin, int nblocks, int C, int G,
+                            int P, top_key_t* top_key, float* colsum, hipStream_t stream) {
+#define SC_F32_CASE(CC, GG, PP, WW)                                                         \
+  if (C == CC && G == GG && P == PP)                                                        \
+    return launch_fused32_typed<CC, GG, WW, In>(x, stride, N, bmin, nblocks, top_key, colsum, \
+                                                stream);
+  SC_F32_CASE(4, 4, 0, 1)
+  SC_F32_CASE(2, 2, 0, 1)
+  SC_F32_CASE(2, 4, 0, 1)
+  SC_F32_CASE(2, 4, 2, 2)
+  SC_F32_CASE(2, 8, 2, 2)
+  SC_F32_CASE(2, 8, 4, 4)
+  SC_F32_CASE(4, 8, 2, 2)
+#undef SC_F32_CASE
+  return hipErrorInvalidValue;
+}
+
 constexpr int kPass1C = 4;
 constexpr int kPass2C = 4;
 
@@ -1335,6 +1531,10 @@ hipError_t tree_blocks_dispatch(const In* x, long stride, int T, int N, int drif
 }  // namespace
 
 bool fused_supported(int T, int C, int G, int P) {
+  if (T == 32)
+    return (P == 0 && ((C == 4 && G == 4) || (C == 2 && (G == 2 || G == 4)))) ||
+           (P == 2 && ((C == 2 && (G == 4 || G == 8)) || (C == 4 && G == 8))) ||
+           (P == 4 && C == 2 && G == 8);
 #define SC_FUSED_SUPPORTED(TT, CC, GG, PP) \
   if (T == TT && C == CC && G == GG && P == PP) return true;
   SC_FUSED_VARIANTS(SC_FUSED_SUPPORTED)
@@ -1346,6 +1546,21 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
                                  int nblocks, int C, int G, int P, top_key_t* top_key,
                                  float* colsum, hipStream_t stream, const int* tile_list,
                                  const int* tile_count) {
+  if (T == 32) {
+    if (tile_list) return hipErrorInvalidValue;
+    switch (dtype) {
+      case 0:
+        return fused32_dispatch<float>((const float*)x, stride, N, bmin, nblocks, C, G, P,
+                                       top_key, colsum, stream);
+      case 1:
+        return fused32_dispatch<__half>((const __half*)x, stride, N, bmin, nblocks, C, G, P,
+                                        top_key, colsum, stream);
+      case 2:
+        return fused32_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, N, bmin,
+                                                nblocks, C, G, P, top_key, colsum, stream);
+    }
+    return hipErrorInvalidValue;
+  }
   switch (dtype) {
     case 0:
       return fused_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, C, G, P,
@@ -1362,6 +1577,7 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
 }
 
 int fused_tile_width(int T, int C) {
+  if (T == 32) return C == 4 ? Tile32<4>::kOut : C == 2 ? Tile32<2>::kOut : 0;
   if (T < 2 || C <= 0) return 0;
   return (kWave - (T - 1 + C - 1) / C) * C;
 }

@@ -190,6 +190,27 @@ def test_two_pass_shapes_are_bit_identical(gpu, oracle, T, shape):
     assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
 
 
+@pytest.mark.parametrize("shape", [(4, 4, 0), (2, 2, 0), (2, 4, 0), (2, 4, 2), (2, 8, 2),
+                                   (2, 8, 4), (4, 8, 2)])
+@pytest.mark.parametrize("N,max_drift", [(7000, 2.0), (1500, 4.0)])
+def test_one_pass_t32_shapes_are_bit_identical(gpu, oracle, shape, N, max_drift):
+    # The one-pass T=32 kernel (default at 32 time steps) for every compiled
+    # shape, including a drift range wider than the channel.
+    T = 32
+    print("one-pass T=32 shape=%s N=%d" % (shape, N), flush=True)
+    rng = np.random.default_rng(N)
+    tones = [(rng.uniform(0, N), rng.uniform(-2, 2), rng.uniform(1, 3)) for _ in range(6)]
+    data = _waterfall(T, N, 77, tones)
+    hits, st, ref = _compare_search(oracle, data, T, -2.79e-6, 18.25, max_drift, 0.0, 5.0,
+                                    tuning=shape)
+    np.testing.assert_array_equal(st["column_sums"], ref.column_sums)
+    np.testing.assert_array_equal(st["top_sums"], ref.top_sums)
+    pos = ref.top_sums > 0
+    np.testing.assert_array_equal(st["top_blocks"][pos], ref.top_blocks[pos])
+    np.testing.assert_array_equal(st["top_offsets"][pos], ref.top_offsets[pos])
+    assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
+
+
 def test_device_stats_mode(gpu, oracle):
     T, N = 16, 1 << 18
     rng = np.random.default_rng(5)
