@@ -234,8 +234,29 @@ sc_status sc_hits_writer_open(const char* filename, const sc_file_meta* meta,
  * the .hits writer for its filterbank slice; may be NULL for .dat). */
 sc_status sc_writer_record_hit(sc_writer* w, const sc_hit* hit, const float* input);
 
-/* Closes the file (and renames .hits.tmp to .hits). */
+/* Replaces EventFileWriter (event_file_writer.cpp:19-69): one Cap'n Proto Event
+ * message (hit.capnp:91-93) per cadence event, written to filename + ".tmp" and
+ * renamed on close. metas: one per cadence file. */
+sc_status sc_events_writer_open(const char* filename, const sc_file_meta* metas, int num_files,
+                                sc_writer** out);
+
+/* EventFileWriter::write: hits[i] is file i's hit or NULL (hits[0] must not be
+ * NULL); inputs[i] the host [T][coarse_channel_size] float32 data of file i's
+ * coarse channel, for the filterbank slices over the event's channel range. */
+sc_status sc_events_writer_record(sc_writer* w, const sc_hit* const* hits,
+                                  const float* const* inputs);
+
+/* Closes the file (and renames .hits.tmp / .events.tmp). */
 sc_status sc_writer_close(sc_writer* w);
+
+/* Replaces findEvents (find_events.cpp:66-227): searches the .fil files of an
+ * ABACAD cadence coarse channel by coarse channel (host-exact statistics) and
+ * writes every event (a hit in each "on" file where the first file's drift
+ * predicts it, none in the "off" files' range) to `output`. Metadata mismatches
+ * are SC_ERR_INVALID_ARGUMENT where the reference exits. */
+sc_status sc_find_events(const char* const* filenames, int num_files, const char* output,
+                         double max_drift, double snr_on, double snr_off, long* num_events,
+                         void* stream);
 
 /* ---- Learned block proposal ------------------------------------------------
  * No counterpart in the reference. A small BF16 convolutional network scores

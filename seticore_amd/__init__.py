@@ -143,6 +143,9 @@ def lib():
                                 P(d), P(d), P(vp)],
         "sc_writer_record_hit": [vp, P(ScHit), P(ctypes.c_float)],
         "sc_writer_close": [vp],
+        "sc_events_writer_open": [ctypes.c_char_p, P(ScFileMeta), i, P(vp)],
+        "sc_events_writer_record": [vp, P(P(ScHit)), P(P(ctypes.c_float))],
+        "sc_find_events": [P(ctypes.c_char_p), i, ctypes.c_char_p, d, d, d, P(lng), vp],
         "sc_pn_create": [i, P(vp)],
         "sc_pn_destroy": [vp],
         "sc_pn_memory_usage": [vp, P(sz)],
@@ -752,3 +755,52 @@ def condition_coarse_channel(data, num_timesteps, has_dc_spike, stream=None):
                                             int(T_r), int(N), int(data.stride(0)),
                                             int(bool(has_dc_spike)), _stream_ptr(stream)))
     return data
+
+
+# ---------------------------------------------------------------------------
+# Cadence events (find_events.cpp, event_file_writer.cpp).
+
+class EventFileWriter:
+    """EventFileWriter (event_file_writer.cpp:19-69): one Event message per
+    cadence event. `metadatas` are FilterbankMetadata objects, one per file."""
+
+    def __init__(self, filename, metadatas):
+        self._keep = [m._sc_file_meta() for m in metadatas]  # keeps the name bytes alive
+        metas = (ScFileMeta * len(metadatas))(*self._keep)
+        h = ctypes.c_void_p()
+        check(lib().sc_events_writer_open(os.fsencode(filename), metas, len(metadatas),
+                                          ctypes.byref(h)))
+        self._h = h
+        self.n = len(metadatas)
+
+    def write(self, hits, inputs):
+        """hits: one DedopplerHit or None per file; inputs: host float32
+        [T][coarse_channel_size] arrays, one per file."""
+        import numpy as np
+        hp = (ctypes.POINTER(ScHit) * self.n)(*[ctypes.pointer(h) if h is not None else None
+                                               for h in hits])
+        arrs = [np.ascontiguousarray(a, np.float32) for a in inputs]
+        ip = (ctypes.POINTER(ctypes.c_float) * self.n)(
+            *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for a in arrs])
+        check(lib().sc_events_writer_record(self._h, hp, ip))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            check(lib().sc_writer_close(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def findEvents(input_filenames, output_filename, max_drift, snr_on, snr_off, stream=None):
+    """find_events.cpp:66-227 over .fil files; returns the number of events."""
+    names = (ctypes.c_char_p * len(input_filenames))(*[os.fsencode(f) for f in input_filenames])
+    n = ctypes.c_long()
+    check(lib().sc_find_events(names, len(input_filenames), os.fsencode(output_filename),
+                               float(max_drift), float(snr_on), float(snr_off), ctypes.byref(n),
+                               _stream_ptr(stream)))
+    return n.value

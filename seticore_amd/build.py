@@ -32,6 +32,7 @@ SOURCES = [
     ("pn_capi.cpp", []),
     ("hit_writers.cpp", []),
     ("fil_reader.cpp", []),
+    ("find_events.cpp", []),
 ]
 
 

@@ -1,6 +1,7 @@
 // Hit output writers: the turboSETI-style .dat text file
-// (dat_file_writer.cpp:15-73) and the .hits stream of Cap'n Proto messages
-// (hit_file_writer.cpp:16-97, schema hit.capnp:1-86).
+// (dat_file_writer.cpp:15-73), the .hits stream of Cap'n Proto messages
+// (hit_file_writer.cpp:16-97, schema hit.capnp:1-86) and the .events stream
+// of cadence events (event_file_writer.cpp:19-69, hit.capnp:91-93).
 //
 // The .hits encoder is written against the Cap'n Proto wire format directly
 // (the capnp library is not a dependency): one unpacked message per hit, laid
@@ -43,18 +44,19 @@ std::string fmt_fixed(double v, int width, int prec) {
 
 // ---- Cap'n Proto wire encoding --------------------------------------------
 
-constexpr int kFirstSegmentWords = 1024;
+constexpr long kFirstSegmentWords = 1024;
 
 struct Message {
   std::vector<std::vector<uint64_t>> segs;
-  Message() {
+  long limit;  // words segment 0 may hold
+  explicit Message(long first_segment_words = kFirstSegmentWords) : limit(first_segment_words) {
     segs.reserve(2);  // references to segs[0] stay valid when segment 1 is added
     segs.emplace_back();
   }
   // Allocates `words` in segment 0 if it fits; returns the word index or -1.
   long alloc0(long words) {
     auto& s = segs[0];
-    if ((long)s.size() + words > kFirstSegmentWords) return -1;
+    if ((long)s.size() + words > limit) return -1;
     long at = (long)s.size();
     s.resize(s.size() + words, 0);
     return at;
@@ -93,17 +95,12 @@ struct HitsMeta {
   bool coherent(int beam) const { return beam >= 0 && beam < (int)beam_names.size(); }
 };
 
-// One message for one hit (hit_file_writer.cpp:34-97).
-std::vector<char> encode_hit(const HitsMeta& m, const sc_hit& h, const float* input) {
-  Message msg;
+// Signal (hit.capnp:11-46, 6 data words) for `h`, pointed to from segment-0
+// word `ptr_word` (hit_file_writer.cpp:54-65).
+void encode_signal(Message& msg, long ptr_word, const sc_hit& h) {
   auto& s0 = msg.segs[0];
-  const long root = msg.alloc0(1);
-  const long hit = msg.alloc0(2);  // Hit: 0 data words, 2 pointers
-  s0[root] = struct_ptr(root, hit, 0, 2);
-
-  // Signal: 6 data words (hit.capnp:11-46).
   const long sig = msg.alloc0(6);
-  s0[hit] = struct_ptr(hit, sig, 6, 0);
+  s0[ptr_word] = struct_ptr(ptr_word, sig, 6, 0);
   put<double>(s0, sig, 0, h.frequency);
   put<int32_t>(s0, sig, 8, h.index);
   put<int32_t>(s0, sig, 12, h.drift_steps);
@@ -114,19 +111,25 @@ std::vector<char> encode_hit(const HitsMeta& m, const sc_hit& h, const float* in
   put<int32_t>(s0, sig, 36, h.num_timesteps);
   put<float>(s0, sig, 40, h.power);
   put<float>(s0, sig, 44, h.incoherent_power);
+}
 
-  // Filterbank: 9 data words + 2 pointers (hit.capnp:50-78).
+// Filterbank (hit.capnp:50-78: 9 data words + 2 pointers) for the channel
+// range around [low, high] of one coarse channel (hit_file_writer.cpp:67-97),
+// pointed to from segment-0 word `ptr_word`. The data goes to a second
+// segment behind a far pointer when segment 0 is full.
+void encode_filterbank(Message& msg, long ptr_word, const HitsMeta& m, int beam,
+                       int coarse_channel, int low, int high, const float* input) {
+  auto& s0 = msg.segs[0];
   const long fb = msg.alloc0(11);
-  s0[hit + 1] = struct_ptr(hit + 1, fb, 9, 2);
-  const int beam = h.beam;
+  s0[ptr_word] = struct_ptr(ptr_word, fb, 9, 2);
   const std::string& name = m.coherent(beam) ? m.beam_names[beam] : m.source_name;
   const double ra = m.coherent(beam) ? m.beam_ras[beam] : m.src_raj;
   const double dec = m.coherent(beam) ? m.beam_decs[beam] : m.src_dej;
   const long extra = 40;  // EXTRA_COLUMNS, hit_file_writer.cpp:16
-  const long begin = std::max((long)low_index(h) - extra, 0L);
-  const long end = std::min((long)high_index(h) + extra, m.coarse_channel_size) - 1;
+  const long begin = std::max((long)low - extra, 0L);
+  const long end = std::min((long)high + extra, m.coarse_channel_size) - 1;
   const long nch = end - begin;
-  const long coarse_offset = (long)h.coarse_channel * m.coarse_channel_size;
+  const long coarse_offset = (long)coarse_channel * m.coarse_channel_size;
   put<double>(s0, fb, 0, m.fch1 + (coarse_offset + begin) * m.foff);
   put<double>(s0, fb, 8, m.foff);
   put<double>(s0, fb, 16, m.tstart);
@@ -136,7 +139,7 @@ std::vector<char> encode_hit(const HitsMeta& m, const sc_hit& h, const float* in
   put<int32_t>(s0, fb, 48, m.telescope_id);
   put<int32_t>(s0, fb, 52, (int32_t)m.num_timesteps);
   put<int32_t>(s0, fb, 56, (int32_t)nch);
-  put<int32_t>(s0, fb, 60, h.coarse_channel);
+  put<int32_t>(s0, fb, 60, coarse_channel);
   put<int32_t>(s0, fb, 64, (int32_t)begin);
   put<int32_t>(s0, fb, 68, beam);
 
@@ -169,8 +172,10 @@ std::vector<char> encode_hit(const HitsMeta& m, const sc_hit& h, const float* in
     s1[0] = list_ptr(0, 1, 4, (uint32_t)n);
     if (n) put_bytes(s1, 1, 0, vals.data(), n * 4);
   }
+}
 
-  // Stream framing: segment count - 1, segment sizes, padding to 8 bytes.
+// Stream framing: segment count - 1, segment sizes, padding to 8 bytes.
+std::vector<char> frame(const Message& msg) {
   const uint32_t nseg = (uint32_t)msg.segs.size();
   std::vector<uint32_t> table;
   table.push_back(nseg - 1);
@@ -185,10 +190,86 @@ std::vector<char> encode_hit(const HitsMeta& m, const sc_hit& h, const float* in
   return out;
 }
 
+// One message for one hit (hit_file_writer.cpp:34-97): root pointer, Hit,
+// Signal, Filterbank, source name, data, in the order the reference builds them.
+std::vector<char> encode_hit(const HitsMeta& m, const sc_hit& h, const float* input) {
+  Message msg;
+  auto& s0 = msg.segs[0];
+  const long root = msg.alloc0(1);
+  const long hit = msg.alloc0(2);  // Hit: 0 data words, 2 pointers
+  s0[root] = struct_ptr(root, hit, 0, 2);
+  encode_signal(msg, hit, h);
+  encode_filterbank(msg, hit + 1, m, h.beam, h.coarse_channel, low_index(h), high_index(h),
+                    input);
+  return frame(msg);
+}
+
+// One message for one cadence event (event_file_writer.cpp:35-69, schema
+// hit.capnp:91-93): an Event with one Hit per file; every Hit has a
+// Filterbank over the event's channel range in that file, and a Signal when
+// the file has a hit. A single segment holds it (any segmentation is valid
+// Cap'n Proto; readers do not depend on it).
+std::vector<char> encode_event(const std::vector<HitsMeta>& metas, const sc_hit* const* hits,
+                               const float* const* inputs) {
+  const long n = (long)metas.size();
+  Message msg(1L << 40);
+  auto& s0 = msg.segs[0];
+  const long root = msg.alloc0(1);
+  const long ev = msg.alloc0(1);  // Event: 0 data words, 1 pointer
+  s0[root] = struct_ptr(root, ev, 0, 1);
+  // List(Hit): inline composite, tag word + n elements of 0 data / 2 pointer words.
+  const long tag = msg.alloc0(1 + 2 * n);
+  s0[ev] = list_ptr(ev, tag, 7, (uint32_t)(2 * n));
+  s0[tag] = ((uint64_t)(uint32_t)(n << 2)) | ((uint64_t)(0u | (2u << 16)) << 32);
+  int low = low_index(*hits[0]), high = high_index(*hits[0]);
+  for (long i = 1; i < n; ++i) {
+    if (!hits[i]) continue;
+    low = std::min(low, low_index(*hits[i]));
+    high = std::max(high, high_index(*hits[i]));
+  }
+  const int beam = hits[0]->beam, coarse_channel = hits[0]->coarse_channel;
+  for (long i = 0; i < n; ++i) {
+    const long elem = tag + 1 + 2 * i;
+    encode_filterbank(msg, elem + 1, metas[i], beam, coarse_channel, low, high, inputs[i]);
+    if (hits[i]) encode_signal(msg, elem, *hits[i]);
+  }
+  return frame(msg);
+}
+
+HitsMeta hits_meta(const sc_file_meta* meta) {
+  HitsMeta m;
+  m.source_name = meta->source_name ? meta->source_name : "";
+  m.fch1 = meta->fch1;
+  m.foff = meta->foff;
+  m.tstart = meta->tstart;
+  m.tsamp = meta->tsamp;
+  m.src_raj = meta->src_raj;
+  m.src_dej = meta->src_dej;
+  m.num_timesteps = meta->num_timesteps;
+  m.coarse_channel_size = meta->coarse_channel_size;
+  m.telescope_id = meta->telescope_id;
+  return m;
+}
+
+sc_status write_all(int fd, const std::vector<char>& bytes) {
+  size_t done = 0;
+  while (done < bytes.size()) {
+    ssize_t k = ::write(fd, bytes.data() + done, bytes.size() - done);
+    if (k < 0) {
+      if (errno == EINTR) continue;
+      return werr("write failed");
+    }
+    done += (size_t)k;
+  }
+  return SC_OK;
+}
+
 }  // namespace
 
 struct sc_writer {
   bool dat = false;
+  bool events = false;
+  std::vector<HitsMeta> event_metas;  // .events: one per cadence file
   // .dat
   FILE* f = nullptr;
   int hit_count = 0;
@@ -263,17 +344,8 @@ sc_status sc_hits_writer_open(const char* filename, const sc_file_meta* meta, in
     return werr("could not open " + std::string(filename) + ".tmp for writing. errno = " +
                 std::to_string(e));
   }
+  w->meta = hits_meta(meta);
   HitsMeta& m = w->meta;
-  m.source_name = meta->source_name ? meta->source_name : "";
-  m.fch1 = meta->fch1;
-  m.foff = meta->foff;
-  m.tstart = meta->tstart;
-  m.tsamp = meta->tsamp;
-  m.src_raj = meta->src_raj;
-  m.src_dej = meta->src_dej;
-  m.num_timesteps = meta->num_timesteps;
-  m.coarse_channel_size = meta->coarse_channel_size;
-  m.telescope_id = meta->telescope_id;
   for (int b = 0; b < num_beams; ++b) {
     m.beam_names.push_back(beam_source_names[b] ? beam_source_names[b] : "");
     m.beam_ras.push_back(beam_ras[b]);
@@ -283,8 +355,39 @@ sc_status sc_hits_writer_open(const char* filename, const sc_file_meta* meta, in
   return SC_OK;
 }
 
+// event_file_writer.cpp:19-28: opened like the .hits writer (no O_TRUNC,
+// renamed on close).
+sc_status sc_events_writer_open(const char* filename, const sc_file_meta* metas, int num_files,
+                                sc_writer** out) {
+  if (!filename || !metas || !out || num_files <= 0) return werr("null argument");
+  sc_writer* w = new sc_writer();
+  w->events = true;
+  w->final_name = filename;
+  w->tmp_name = w->final_name + ".tmp";
+  w->fd = open(w->tmp_name.c_str(), O_WRONLY | O_CREAT, 0664);
+  if (w->fd < 0) {
+    int e = errno;
+    delete w;
+    return werr("could not open " + std::string(filename) + ".tmp for writing. errno = " +
+                std::to_string(e));
+  }
+  for (int i = 0; i < num_files; ++i) w->event_metas.push_back(hits_meta(&metas[i]));
+  *out = w;
+  return SC_OK;
+}
+
+sc_status sc_events_writer_record(sc_writer* w, const sc_hit* const* hits,
+                                  const float* const* inputs) {
+  if (!w || !hits || !inputs || !w->events) return werr("not an events writer");
+  if (!hits[0]) return werr("the first file of an event must have a hit");
+  for (size_t i = 0; i < w->event_metas.size(); ++i)
+    if (!inputs[i]) return werr("events need every file's coarse channel data");
+  return write_all(w->fd, encode_event(w->event_metas, hits, inputs));
+}
+
 sc_status sc_writer_record_hit(sc_writer* w, const sc_hit* hit, const float* input) {
   if (!w || !hit) return werr("null argument");
+  if (w->events) return werr("use sc_events_writer_record for an events writer");
   if (w->dat) {
     // dat_file_writer.cpp:58-73
     ++w->hit_count;
@@ -301,17 +404,7 @@ sc_status sc_writer_record_hit(sc_writer* w, const sc_hit* hit, const float* inp
     return SC_OK;
   }
   if (!input) return werr(".hits needs the coarse channel's input data");
-  std::vector<char> bytes = encode_hit(w->meta, *hit, input);
-  size_t done = 0;
-  while (done < bytes.size()) {
-    ssize_t k = ::write(w->fd, bytes.data() + done, bytes.size() - done);
-    if (k < 0) {
-      if (errno == EINTR) continue;
-      return werr("write failed");
-    }
-    done += (size_t)k;
-  }
-  return SC_OK;
+  return write_all(w->fd, encode_hit(w->meta, *hit, input));
 }
 
 sc_status sc_writer_close(sc_writer* w) {

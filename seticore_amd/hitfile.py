@@ -1,5 +1,6 @@
-"""Reader for .hits files: a stream of unpacked Cap'n Proto messages with the
-schema of hit.capnp:1-86 (Hit{signal: Signal, filterbank: Filterbank}).
+"""Reader for .hits and .events files: streams of unpacked Cap'n Proto messages
+with the schema of hit.capnp (Hit{signal: Signal, filterbank: Filterbank},
+Event{hits: List(Hit)}).
 
 Plays the role of the reference's hitls tool (hitls.cpp:19-58). Pure Python,
 no capnp dependency. Struct sizes are read from the pointers, so files written
@@ -92,9 +93,14 @@ def read_messages(data):
 
 def decode_hit(segments):
     """Decodes one Hit message into {'signal': {...}, 'filterbank': {...}}."""
-    import numpy as np
     msg = _Msg(segments)
-    root = msg.struct_at(0, 0)
+    out = _decode_hit_struct(msg, msg.struct_at(0, 0))
+    out["layout"] = {"segments": [len(x) // 8 for x in segments]}
+    return out
+
+
+def _decode_hit_struct(msg, root):
+    import numpy as np
     s, t, dw, pw = root
     out = {"signal": None, "filterbank": None}
     sig = msg.struct_at(s, t + dw + 0) if pw > 0 else None
@@ -119,8 +125,33 @@ def decode_hit(segments):
         else:
             f["data"] = np.zeros(0, np.float32)
         out["filterbank"] = f
-    out["layout"] = {"segments": [len(x) // 8 for x in segments]}
     return out
+
+
+def decode_event(segments):
+    """Decodes one Event message (hit.capnp:91-93) into a list of hits, one per
+    cadence file ({'signal': {...} or None, 'filterbank': {...}})."""
+    msg = _Msg(segments)
+    s, t, dw, pw = msg.struct_at(0, 0)
+    ls, lt, esize, words = msg.list_at(s, t + dw)
+    assert esize == 7  # inline composite
+    tag = msg.word(ls, lt)
+    count = (tag & 0xFFFFFFFF) >> 2
+    edw = (tag >> 32) & 0xFFFF
+    epw = (tag >> 48) & 0xFFFF
+    assert count * (edw + epw) == words
+    return [_decode_hit_struct(msg, (ls, lt + 1 + i * (edw + epw), edw, epw))
+            for i in range(count)]
+
+
+def read_events(path_or_bytes):
+    """Decodes every event in a .events file (path) or byte string."""
+    if isinstance(path_or_bytes, (bytes, bytearray)):
+        data = bytes(path_or_bytes)
+    else:
+        with open(path_or_bytes, "rb") as f:
+            data = f.read()
+    return [decode_event(segs) for segs in read_messages(data)]
 
 
 def read_hits(path_or_bytes):
