@@ -61,10 +61,13 @@ def synth(x, seed, recipe):
     return x
 
 
-def block_labels(hits, num_blocks):
+def block_labels(hits, num_blocks, label_snr=0.0):
+    """1 for blocks holding the start channel of a plain-search hit with
+    SNR >= label_snr, else 0."""
     y = np.zeros(num_blocks, np.float32)
     for h in hits:
-        y[h.index // BLOCK_CHANNELS] = 1.0
+        if h.snr >= label_snr:
+            y[h.index // BLOCK_CHANNELS] = 1.0
     return y
 
 
@@ -78,9 +81,10 @@ class Dataset:
         self.recipes = recipes
 
 
-def make_dataset(num_cc, N, seed, **recipe_args):
+def make_dataset(num_cc, N, seed, label_snr=0.0, **recipe_args):
     """num_cc synthetic coarse channels of N channels: each is searched with
-    the plain search (device statistics) for its labels, then pooled."""
+    the plain search (device statistics) for its labels (hits with SNR >=
+    label_snr), then pooled."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     nblk = N // BLOCK_CHANNELS
@@ -100,7 +104,7 @@ def make_dataset(num_cc, N, seed, **recipe_args):
         st = dd.state()
         stats = torch.tensor([st["median"], st["std"]], dtype=torch.float32, device="cuda")
         ProposalNet.pool(x, stats, SC_F32, out=tiles[cc * nblk:(cc + 1) * nblk])
-        labels[cc * nblk:(cc + 1) * nblk] = block_labels(hits, nblk)
+        labels[cc * nblk:(cc + 1) * nblk] = block_labels(hits, nblk, label_snr)
         all_hits.append(hits)
         recipes.append(recipe)
     torch.cuda.synchronize()
