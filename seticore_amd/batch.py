@@ -89,7 +89,10 @@ class Batch:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None:
-            _lib().sc_batch_destroy(h)
+            try:
+                _lib().sc_batch_destroy(h)
+            except Exception:  # interpreter shutdown: the library is already gone
+                pass
             self._h = None
 
     def run(self, stream=None):

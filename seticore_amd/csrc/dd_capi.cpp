@@ -1025,23 +1025,30 @@ extern "C" sc_status sc_condition_coarse_channel(float* data, int num_timesteps,
 // beamforming_pipeline.cpp:145-253). Here every instance's launch sequence
 // (reset, path sums, device statistics, window merge, candidate copy) is
 // captured once into a single hipGraph and the whole batch runs as one graph
-// launch. Instances of one shape (rounded time steps, channels, foff, tsamp)
-// share a search handle - their device state is reused in sequence, and each
-// instance's candidates land in its own pinned host area.
+// launch. The path sums of successive instances form one chain; each
+// instance's statistics/merge/copy tail hangs off its path sums on a second
+// chain, so it can overlap the next instance's path sums. Instances of one
+// shape (rounded time steps, channels, foff, tsamp) share search handles:
+// two, used alternately, for shapes up to 32 time steps (where the tail is a
+// noticeable share of a search), one otherwise; an instance's path sums wait
+// for the tail of the previous instance on the same handle. Each instance's
+// candidates land in its own pinned host area.
 
 struct sc_batch {
   struct Instance {
     sc_instance in;
     DriftRange r;
-    int shape = 0;
+    int shape = 0;   // index into shapes (the handle the instance runs on)
     size_t cand_offset = 0;  // into h_cand
     std::vector<sc_hit> hits;
   };
   std::vector<Instance> inst;
-  std::vector<sc_dd*> shapes;
+  std::vector<sc_dd*> shapes;   // handles; a shape's alternates are consecutive
+  std::vector<int> alternates;  // per handle: handles of its shape (1 or 2)
   int4* h_cand = nullptr;
   float* h_stats = nullptr;  // 2 per instance
   hipStream_t capture_stream = nullptr;
+  hipStream_t tail_stream = nullptr;
   hipGraphExec_t exec = nullptr;
   int nodes = 0;
 };
@@ -1051,6 +1058,7 @@ namespace {
 void batch_release(sc_batch* b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->capture_stream) hipStreamDestroy(b->capture_stream);
+  if (b->tail_stream) hipStreamDestroy(b->tail_stream);
   for (sc_dd* dd : b->shapes) sc_dd_destroy(dd);
   hipHostFree(b->h_cand);
   hipHostFree(b->h_stats);
@@ -1085,27 +1093,38 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
     for (size_t k = 0; k < b->shapes.size(); ++k) {
       const sc_dd* dd = b->shapes[k];
       if (dd->rounded == T && dd->num_channels == in.num_channels && dd->foff == in.foff &&
-          dd->tsamp == in.tsamp)
+          dd->tsamp == in.tsamp) {
         shape = (int)k;
+        break;
+      }
     }
     if (shape < 0) {
-      sc_dd* dd = nullptr;
-      s = sc_dd_create(in.num_timesteps, in.num_channels, in.foff, in.tsamp, 0, &dd);
-      if (s != SC_OK) break;
-      dd->stats_mode = SC_STATS_DEVICE;
-      b->shapes.push_back(dd);
-      shape = (int)b->shapes.size() - 1;
-      for (int k = 0; k < num_tuning; ++k) {
-        const sc_tuning& t = tuning[k];
-        if (t.rounded_timesteps != T || t.num_channels != in.num_channels) continue;
-        if (T <= 16 || (T == 32 && t.pass_group == 0))
-          s = sc_dd_set_tuning(dd, t.channels_per_lane, t.drift_group, t.prefetch);
-        else if (sc::two_pass_supported(T, t.pass_group))
-          s = sc_dd_set_two_pass(dd, t.pass_group, t.pass_blocks);
-        break;
+      shape = (int)b->shapes.size();
+      const int copies = T <= 32 ? 2 : 1;
+      for (int c = 0; c < copies && s == SC_OK; ++c) {
+        sc_dd* dd = nullptr;
+        s = sc_dd_create(in.num_timesteps, in.num_channels, in.foff, in.tsamp, 0, &dd);
+        if (s != SC_OK) break;
+        dd->stats_mode = SC_STATS_DEVICE;
+        b->shapes.push_back(dd);
+        b->alternates.push_back(copies);
+        for (int k = 0; k < num_tuning; ++k) {
+          const sc_tuning& t = tuning[k];
+          if (t.rounded_timesteps != T || t.num_channels != in.num_channels) continue;
+          if (T <= 16 || (T == 32 && t.pass_group == 0))
+            s = sc_dd_set_tuning(dd, t.channels_per_lane, t.drift_group, t.prefetch);
+          else if (sc::two_pass_supported(T, t.pass_group))
+            s = sc_dd_set_two_pass(dd, t.pass_group, t.pass_blocks);
+          break;
+        }
       }
       if (s != SC_OK) break;
     }
+    // Alternate over the shape's handles in instance order.
+    int used = 0;
+    for (const auto& prev : b->inst)
+      if (prev.shape >= shape && prev.shape < shape + b->alternates[shape]) ++used;
+    shape += used % b->alternates[shape];
     bi.shape = shape;
     bi.r = drift_range(b->shapes[shape], in.max_drift);
     if (bi.r.window <= 0 || (long)(bi.r.bmax - bi.r.bmin + 1) * T > 0x1FFFFFFFL) {
@@ -1121,22 +1140,48 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
     s = host_alloc((void**)&b->h_stats, 2 * sizeof(float) * (size_t)count, "statistics");
   if (s == SC_OK) {
     hipError_t e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->tail_stream, hipStreamNonBlocking);
     if (e != hipSuccess) s = hip_fail(e, "stream");
   }
+  std::vector<hipEvent_t> events;
+  auto new_event = [&](hipEvent_t* ev) {
+    hipError_t e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    if (e == hipSuccess) events.push_back(*ev);
+    return e;
+  };
   if (s == SC_OK) {
     hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) s = hip_fail(e, "begin capture");
+    // last_tail[h]: the tail event of the previous instance on handle h.
+    std::vector<hipEvent_t> last_tail(b->shapes.size(), nullptr);
+    hipEvent_t tail_done = nullptr;
     for (size_t i = 0; i < b->inst.size() && s == SC_OK && e == hipSuccess; ++i) {
       sc_batch::Instance& bi = b->inst[i];
       sc_dd* dd = b->shapes[bi.shape];
+      if (last_tail[bi.shape]) e = hipStreamWaitEvent(b->capture_stream, last_tail[bi.shape], 0);
+      if (e != hipSuccess) break;
       s = enqueue_path_sums(dd, bi.in.data, bi.in.dtype, bi.in.row_stride, bi.r,
                             b->capture_stream);
-      if (s == SC_OK)
-        s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, b->capture_stream,
-                                b->h_cand + bi.cand_offset, b->h_stats + 2 * i);
+      if (s != SC_OK) break;
+      hipEvent_t sums = nullptr, tail = nullptr;
+      e = new_event(&sums);
+      if (e == hipSuccess) e = new_event(&tail);
+      if (e == hipSuccess) e = hipEventRecord(sums, b->capture_stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->tail_stream, sums, 0);
+      if (e != hipSuccess) break;
+      s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, b->tail_stream,
+                              b->h_cand + bi.cand_offset, b->h_stats + 2 * i);
+      if (s != SC_OK) break;
+      e = hipEventRecord(tail, b->tail_stream);
+      last_tail[bi.shape] = tail;
+      tail_done = tail;
     }
+    // Join the tail chain back into the capturing stream.
+    if (e == hipSuccess && s == SC_OK && tail_done)
+      e = hipStreamWaitEvent(b->capture_stream, tail_done, 0);
+    if (e != hipSuccess && s == SC_OK) s = hip_fail(e, "capture dependencies");
     hipGraph_t graph = nullptr;
-    if (e == hipSuccess) {
+    {
       hipError_t e2 = hipStreamEndCapture(b->capture_stream, &graph);
       if (s == SC_OK && e2 != hipSuccess) s = hip_fail(e2, "end capture");
     }
@@ -1149,6 +1194,7 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
     }
     if (graph) hipGraphDestroy(graph);
   }
+  for (hipEvent_t ev : events) hipEventDestroy(ev);
   if (s != SC_OK) {
     batch_release(b);
     delete b;
@@ -1186,7 +1232,11 @@ sc_status sc_batch_copy_hits(const sc_batch* b, int instance, sc_hit* hits, size
 
 sc_status sc_batch_info(const sc_batch* b, int* num_shapes, int* graph_nodes) {
   if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
-  if (num_shapes) *num_shapes = (int)b->shapes.size();
+  if (num_shapes) {
+    int n = 0;
+    for (size_t k = 0; k < b->shapes.size(); k += b->alternates[k]) ++n;
+    *num_shapes = n;
+  }
   if (graph_nodes) *graph_nodes = b->nodes;
   return SC_OK;
 }
