@@ -25,7 +25,9 @@ import os
 __version__ = "0.1.0"
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY_PATH = os.path.join(_HERE, "libseticore_amd.so")
+# The in-tree build; SETICORE_AMD_LIBRARY selects another build of the same
+# sources (for comparing kernel variants).
+LIBRARY_PATH = os.environ.get("SETICORE_AMD_LIBRARY") or os.path.join(_HERE, "libseticore_amd.so")
 
 NO_BEAM = -1  # dedoppler_hit.h:9
 

@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(kWave * W)
 // drift block b_first + gi, starting (k*q) & 3 floats in. Entries that no
 // valid path reads are not written.
 
-template <int C, int G, int gp>
+template <int C, int G, bool NT, int gp>
 __device__ __forceinline__ void level16_block(lds_float* const (&rp)[16], int g, int ng, int b0,
                                               int gi0, int k, int c_w, int N, int M,
                                               long row_pitch, float* __restrict__ scratch) {
@@ -625,7 +625,12 @@ __device__ __forceinline__ void level16_block(lds_float* const (&rp)[16], int g,
           float* dst = base + (long)q * M * row_pitch;
           if (col >= 0 && col + C <= row_pitch) {
             if constexpr (C == 4) {
-              *(float4*)(dst + col) = make_float4(v[q][0], v[q][1], v[q][2], v[q][3]);
+              typedef float f4a __attribute__((ext_vector_type(4)));
+              const f4a w = {v[q][0], v[q][1], v[q][2], v[q][3]};
+              if constexpr (NT)
+                __builtin_nontemporal_store(w, (f4a*)(dst + col));
+              else
+                *(f4a*)(dst + col) = w;
             } else {
 #pragma unroll
               for (int j = 0; j < C; ++j) dst[col + j] = v[q][j];
@@ -638,14 +643,16 @@ __device__ __forceinline__ void level16_block(lds_float* const (&rp)[16], int g,
         }
       }
     }
-    level16_block<C, G, gp + 1>(rp, g, ng, b0, gi0, k, c_w, N, M, row_pitch, scratch);
+    level16_block<C, G, NT, gp + 1>(rp, g, ng, b0, gi0, k, c_w, N, M, row_pitch, scratch);
   }
 }
 
 // Pass 1. One 64-lane workgroup per (channel tile, time block k, group of G
 // drift blocks): the 16-row window of time block k is staged in LDS once for
-// the group, as in fused_tree_top_kernel.
-template <int C, int G, typename In>
+// the group, as in fused_tree_top_kernel. NT: streaming (nontemporal) scratch
+// stores, and loads in the second pass - measured faster at T_r = 64, slower
+// at 128 and 256.
+template <int C, int G, bool NT, typename In>
 __global__ void __launch_bounds__(kWave)
     level16_kernel(const In* __restrict__ x, long stride, int N, int M, int b_first, int nb,
                    int ntiles, int total_items, long row_pitch, float* __restrict__ scratch) {
@@ -669,7 +676,7 @@ __global__ void __launch_bounds__(kWave)
   lds_float* rp[16];
   row_bases<16, C, G>(win, rp);
   for (int g = 0; g < ng; g += C) {
-    level16_block<C, G, 0>(rp, g, ng, b0, gi0, k, c_w, N, M, row_pitch, scratch);
+    level16_block<C, G, NT, 0>(rp, g, ng, b0, gi0, k, c_w, N, M, row_pitch, scratch);
 #pragma unroll
     for (int t = 1; t < 16; ++t) rp[t] += t;
   }
@@ -706,7 +713,12 @@ __global__ void __launch_bounds__(kWave * kUpperWaves)
     for (int k = 0; k < M; ++k) {
       if constexpr (C == 4) {
         typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
-        const f4u w = *(const f4u*)(src + (long)k * row_pitch + ((k * q) & 3));
+        const f4u* a = (const f4u*)(src + (long)k * row_pitch + ((k * q) & 3));
+        f4u w;
+        if constexpr (M <= 4)
+          w = __builtin_nontemporal_load(a);
+        else
+          w = *a;
         v[k][0] = w.x;
         v[k][1] = w.y;
         v[k][2] = w.z;
@@ -1474,7 +1486,7 @@ hipError_t two_pass_typed(const In* x, long stride, int N, int bmin, int b_first
   using Sh2 = TileShape<M, kPass2C>;
   const long total2 = ((long)N + Sh2::kOut - 1) / Sh2::kOut;
   if (total1 > (1L << 30) || total2 > (1L << 30)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((level16_kernel<kPass1C, G, In>), dim3((int)((total1 + 7) / 8 * 8)),
+  hipLaunchKernelGGL((level16_kernel<kPass1C, G, (M <= 4), In>), dim3((int)((total1 + 7) / 8 * 8)),
                      dim3(kWave), 0, stream, x, stride, N, M, b_first, nb, (int)ntiles1,
                      (int)total1, row_pitch, scratch);
   hipLaunchKernelGGL((upper_tree_kernel<M, kPass2C>), dim3((int)((total2 + 7) / 8 * 8)),
