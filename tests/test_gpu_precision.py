@@ -44,6 +44,29 @@ def _rfi_waterfall(torch, seed):
     return x
 
 
+@pytest.mark.parametrize("T_r", [32, 64])
+@pytest.mark.parametrize("dtype", [sc.SC_F16, sc.SC_BF16])
+def test_half_storage_parity_longer_windows(gpu, oracle, dtype, T_r):
+    # The one-pass (32) and two-pass (64) kernels on half-width samples equal
+    # the oracle on the same samples widened to float32.
+    import torch
+    N = 1 << 15
+    x = torch.empty((T_r, N), dtype=torch.float32, device="cuda")
+    sc.gen_waterfall(x, T_r, seed=T_r, tones=[(9000.0, 0.7, 3.0), (20000.0, -0.4, 2.0)])
+    xn, _, _ = sc.normalize(x, dtype)
+    widened = xn.float().cpu().numpy()
+    dd = sc.Dedopplerer(T_r, N, FOFF, TSAMP)
+    hits = []
+    dd.search(sc.FilterbankBuffer(T_r, N, data=xn, dtype=dtype),
+              sc.FilterbankMetadata(coarse_channel_size=N), -1, 0, 2.0, 0.0, 6.0, hits)
+    st = dd.state()
+    want = oracle.search(widened, T_r, FOFF, TSAMP, 2.0, 0.0, 6.0, num_threads=8)
+    np.testing.assert_array_equal(st["column_sums"], want.column_sums)
+    np.testing.assert_array_equal(st["top_sums"], want.top_sums)
+    assert [(h.index, h.drift_steps, h.snr) for h in hits] == \
+        [(h["index"], h["drift_steps"], h["snr"]) for h in want.hits]
+
+
 @pytest.mark.parametrize("dtype", [sc.SC_F16, sc.SC_BF16])
 def test_half_storage_tree_parity(gpu, oracle, dtype):
     import torch
