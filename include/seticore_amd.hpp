@@ -10,6 +10,7 @@
 //                        memoryUsage, print_hits)
 //   HitRecorder, DatFileWriter, HitFileWriter, makeHitRecorder
 //                       (hit_recorder.h, dat_file_writer.h, hit_file_writer.h)
+//   FilReader, findEvents (fil_reader.h, find_events.h)
 // Where the reference asserts or exits, the façade throws std::runtime_error.
 // Header-only; link with -lseticore_amd.
 #pragma once
@@ -299,6 +300,58 @@ inline std::unique_ptr<HitRecorder> makeHitRecorder(const std::string& filename,
     return std::unique_ptr<HitRecorder>(new DatFileWriter(filename, metadata, max_drift));
   if (ends_with(".hits")) return std::unique_ptr<HitRecorder>(new HitFileWriter(filename, metadata));
   throw std::runtime_error("could not recognize the file type of " + filename);
+}
+
+// fil_reader.h: a sigproc filterbank file with the reference's inferred
+// metadata; loadCoarseChannel fills a device-visible buffer with zero padding
+// and the DC-spike fix (the reference leaves it unimplemented,
+// fil_reader.cpp:151-154).
+class FilReader : public FilterbankMetadata {
+ public:
+  const std::string filename;
+
+  explicit FilReader(const std::string& filename) : filename(filename) {
+    check(sc_fil_open(filename.c_str(), &f_));
+    sc_fil_meta m;
+    check(sc_fil_metadata(f_, &m));
+    source_name = m.source_name;
+    fch1 = m.fch1;
+    foff = m.foff;
+    tstart = m.tstart;
+    tsamp = m.tsamp;
+    src_raj = m.src_raj;
+    src_dej = m.src_dej;
+    telescope_id = m.telescope_id;
+    num_timesteps = m.num_timesteps;
+    num_channels = m.num_channels;
+    coarse_channel_size = m.coarse_channel_size;
+    num_coarse_channels = m.num_coarse_channels;
+    has_dc_spike = m.has_dc_spike != 0;
+  }
+  ~FilReader() { sc_fil_close(f_); }
+  FilReader(const FilReader&) = delete;
+  FilReader& operator=(const FilReader&) = delete;
+
+  void loadCoarseChannel(int i, FilterbankBuffer* buffer, hipStream_t stream = nullptr) const {
+    if (buffer->num_channels != coarse_channel_size)
+      throw std::runtime_error("FilReader::loadCoarseChannel: buffer width mismatch");
+    check(sc_fil_load_coarse_channel(f_, i, buffer->num_timesteps, buffer->device_data, stream));
+  }
+
+ private:
+  sc_fil* f_ = nullptr;
+};
+
+// find_events.cpp:66-227 over .fil files; returns the number of events written.
+inline long findEvents(const std::vector<std::string>& input_filenames,
+                       const std::string& output_filename, double max_drift, double snr_on,
+                       double snr_off) {
+  std::vector<const char*> names;
+  for (const auto& f : input_filenames) names.push_back(f.c_str());
+  long n = 0;
+  check(sc_find_events(names.data(), (int)names.size(), output_filename.c_str(), max_drift,
+                       snr_on, snr_off, &n, nullptr));
+  return n;
 }
 
 }  // namespace seticore_amd
