@@ -114,7 +114,9 @@ sc_status sc_dd_set_two_pass(sc_dd* dd, int drift_group, int blocks_per_launch);
 
 /* Whether device-statistics searches replay a cached hipGraph of their
  * launch sequence (one graph per input pointer and search parameters, up to
- * 256; captured on first use). Default on. No counterpart in the reference. */
+ * 256; captured the second time the same search is made, so a caller that
+ * passes a new buffer every time never captures). Default on. No counterpart
+ * in the reference. */
 sc_status sc_dd_set_graphs(sc_dd* dd, int enable);
 
 /* Kernel timing for measurement (no counterpart in the reference, which only

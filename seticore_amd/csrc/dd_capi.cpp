@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -139,6 +140,10 @@ struct sc_dd {
   bool graphs = true;
   hipStream_t capture_stream = nullptr;
   std::vector<GraphEntry> graph_cache;
+  // Searches seen once (input pointer and parameters, as a hash): a graph is
+  // captured only when the same search comes again, so callers that hand a
+  // fresh buffer to every search never pay for captures.
+  std::vector<size_t> seen_once;
   static constexpr size_t kMaxGraphs = 256;
 
   // Optional kernel timing (HIP events on the search stream).
@@ -693,7 +698,32 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
     dd->last_std = std_dev;
     host_merge(dd, r, meta, beam, coarse_channel, min_drift, snr_threshold, median, std_dev);
   } else {
-    if (dd->graphs && !dd->timing) {
+    bool use_graph = dd->graphs && !dd->timing;
+    if (use_graph) {
+      size_t h = std::hash<const void*>()(data);
+      for (size_t v : {(size_t)dtype, (size_t)row_stride, (size_t)(long)r.bmin,
+                       (size_t)(long)r.bmax, (size_t)r.window,
+                       std::hash<double>()(snr_threshold), (size_t)dd->lanes,
+                       (size_t)dd->group, (size_t)dd->prefetch, (size_t)dd->pass_group,
+                       (size_t)pass_blocks(dd), (size_t)dd->use_two_pass})
+        h = h * 1000003u ^ v;
+      bool cached = false;
+      for (const auto& g : dd->graph_cache)
+        cached = cached || (g.data == data && g.dtype == dtype && g.stride == row_stride &&
+                            g.bmin == r.bmin && g.window == r.window && g.snr == snr_threshold);
+      if (!cached) {
+        auto it = std::find(dd->seen_once.begin(), dd->seen_once.end(), h);
+        if (it == dd->seen_once.end()) {
+          use_graph = false;  // first time: launch directly, capture next time
+          dd->seen_once.push_back(h);
+          if (dd->seen_once.size() > 4 * sc_dd::kMaxGraphs)
+            dd->seen_once.erase(dd->seen_once.begin());
+        } else {
+          dd->seen_once.erase(it);
+        }
+      }
+    }
+    if (use_graph) {
       hipGraphExec_t exec = nullptr;
       s = search_graph(dd, data, dtype, row_stride, r, snr_threshold, &exec);
       if (s != SC_OK) return s;
