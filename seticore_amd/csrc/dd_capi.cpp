@@ -139,6 +139,9 @@ struct sc_dd {
   };
   bool graphs = true;
   hipStream_t capture_stream = nullptr;
+  // Second stream and fork/join events: the statistics run beside the path sums.
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<GraphEntry> graph_cache;
   // Searches seen once (input pointer and parameters, as a hash): a graph is
   // captured only when the same search comes again, so callers that hand a
@@ -209,6 +212,9 @@ void release(sc_dd* dd) {
   for (auto& g : dd->graph_cache) hipGraphExecDestroy(g.exec);
   dd->graph_cache.clear();
   if (dd->capture_stream) hipStreamDestroy(dd->capture_stream);
+  if (dd->aux_stream) hipStreamDestroy(dd->aux_stream);
+  if (dd->ev_fork) hipEventDestroy(dd->ev_fork);
+  if (dd->ev_join) hipEventDestroy(dd->ev_join);
 }
 
 struct DriftRange {
@@ -443,6 +449,12 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
     s = ensure_tree_scratch(dd);
   }
   if (s == SC_OK) {
+    hipError_t e = hipStreamCreateWithFlags(&dd->aux_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&dd->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&dd->ev_join, hipEventDisableTiming);
+    if (e != hipSuccess) s = hip_fail(e, "statistics stream");
+  }
+  if (s == SC_OK) {
     // Block/offset arrays persist across searches like the reference's
     // (dedoppler.cu:205 only resets the sums); start them at zero.
     hipError_t e = hipMemset(dd->top_blk, 0, N * sizeof(int));
@@ -518,21 +530,41 @@ namespace {
 
 // Reset, then the path sums of every drift block into the top keys, plus the
 // column sums (dedoppler.cu:205-227).
+//
+// With overlap_stats (device statistics, one search at a time): the column
+// sums come first from their own kernel, and the statistics run on the
+// handle's second stream while the path-sum kernels run on `stream` (they
+// need only the column sums); `stream` waits for them at the end, so the
+// tail is then just the window merge.
 sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_stride,
-                            const DriftRange& r, hipStream_t stream) {
+                            const DriftRange& r, hipStream_t stream,
+                            bool overlap_stats = false) {
   const int T = dd->rounded, N = dd->num_channels;
   const int nblocks = r.bmax - r.bmin + 1;
   // dedoppler.cu:205 (the reference zeroes the float sums; the key 0 means "no
   // positive path"), plus the statistics histograms.
   SC_HIP(sc::launch_reset(dd->top_key, N, dd->stats_hist, stream), "reset");
+  const bool one_pass = (T <= 16 || (T == 32 && !dd->use_two_pass)) &&
+                        sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch);
+  if (overlap_stats) {
+    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
+    SC_HIP(hipEventRecord(dd->ev_fork, stream), "fork");
+    SC_HIP(hipStreamWaitEvent(dd->aux_stream, dd->ev_fork, 0), "fork");
+    SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
+                                   dd->stats_out, dd->aux_stream),
+           "device stats");
+    SC_HIP(hipEventRecord(dd->ev_join, dd->aux_stream), "join");
+  }
   if (dd->timing) SC_HIP(hipEventRecord(dd->ev_start, stream), "timing event");
-  if ((T <= 16 || (T == 32 && !dd->use_two_pass)) &&
-      sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch)) {
+  if (one_pass) {
     SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
-                                     dd->group, dd->prefetch, dd->top_key, dd->colsum, stream),
+                                     dd->group, dd->prefetch, dd->top_key,
+                                     overlap_stats ? nullptr : dd->colsum, stream),
            "fused tree");
   } else if (sc::two_pass_supported(T, dd->pass_group)) {
-    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
+    if (!overlap_stats)
+      SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream),
+             "column sums");
     const int nb = pass_blocks(dd);
     sc_status s = ensure_pass_scratch(dd, nb);
     if (s != SC_OK) return s;
@@ -543,7 +575,9 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
              "two-pass tree");
     }
   } else {
-    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
+    if (!overlap_stats)
+      SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream),
+             "column sums");
     for (int b = r.bmin; b <= r.bmax; ++b) {
       float* sums = nullptr;
       sc_status s = run_tree(dd, data, dtype, row_stride, b, &sums, stream);
@@ -552,19 +586,21 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
     }
   }
   if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
+  if (overlap_stats) SC_HIP(hipStreamWaitEvent(stream, dd->ev_join, 0), "join");
   return SC_OK;
 }
 
 // Device statistics, window merge and the copies of the per-window candidates.
 sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_threshold,
                               hipStream_t stream, int4* h_cand = nullptr,
-                              float* h_stats = nullptr) {
+                              float* h_stats = nullptr, bool stats_done = false) {
   if (!h_cand) h_cand = dd->h_cand;
   if (!h_stats) h_stats = dd->h_stats;
   const int T = dd->rounded, N = dd->num_channels;
-  SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
-                                 dd->stats_out, stream),
-         "device stats");
+  if (!stats_done)
+    SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
+                                   dd->stats_out, stream),
+           "device stats");
   SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
                                  dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off, dd->cand,
                                  stream),
@@ -595,8 +631,9 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
     SC_HIP(hipStreamCreateWithFlags(&dd->capture_stream, hipStreamNonBlocking), "stream");
   SC_HIP(hipStreamBeginCapture(dd->capture_stream, hipStreamCaptureModeThreadLocal),
          "begin capture");
-  sc_status s = enqueue_path_sums(dd, data, dtype, row_stride, r, dd->capture_stream);
-  if (s == SC_OK) s = enqueue_device_tail(dd, r, snr_threshold, dd->capture_stream);
+  sc_status s = enqueue_path_sums(dd, data, dtype, row_stride, r, dd->capture_stream, true);
+  if (s == SC_OK)
+    s = enqueue_device_tail(dd, r, snr_threshold, dd->capture_stream, nullptr, nullptr, true);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(dd->capture_stream, &graph);
   if (s != SC_OK) {
@@ -729,8 +766,8 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
       if (s != SC_OK) return s;
       SC_HIP(hipGraphLaunch(exec, stream), "graph launch");
     } else {
-      s = enqueue_path_sums(dd, data, dtype, row_stride, r, stream);
-      if (s == SC_OK) s = enqueue_device_tail(dd, r, snr_threshold, stream);
+      s = enqueue_path_sums(dd, data, dtype, row_stride, r, stream, true);
+      if (s == SC_OK) s = enqueue_device_tail(dd, r, snr_threshold, stream, nullptr, nullptr, true);
       if (s != SC_OK) return s;
     }
     SC_HIP(hipStreamSynchronize(stream), "search");
