@@ -72,6 +72,27 @@ def instance(data, num_timesteps, foff, tsamp, max_drift, min_drift, snr_thresho
                       float(snr_threshold))
 
 
+def multibeam_instances(multibeam, num_timesteps, coarse_channel_size, foff, tsamp, max_drift,
+                        min_drift, snr_threshold, metadata=None, beams=None):
+    """Instances for every (beam, coarse channel) region of a CUDA multibeam
+    buffer [beams][T_r][num_channels] - the regions MultibeamBuffer::
+    copyRegionAsync copies out one by one (multibeam_buffer.cu:65-76,
+    beamforming_pipeline.cpp:230-245); here each instance reads its region in
+    place (row stride = num_channels), so no copy is made. Beam numbers are
+    the buffer's indices (or `beams`)."""
+    B_, T_r, n = multibeam.shape
+    if n % coarse_channel_size:
+        raise ValueError("coarse_channel_size must divide the channel count")
+    out = []
+    for beam in range(B_):
+        for cc in range(n // coarse_channel_size):
+            region = multibeam[beam, :, cc * coarse_channel_size:(cc + 1) * coarse_channel_size]
+            out.append(instance(region, num_timesteps, foff, tsamp, max_drift, min_drift,
+                                snr_threshold, beam=beams[beam] if beams else beam,
+                                coarse_channel=cc, metadata=metadata))
+    return out
+
+
 class Batch:
     """sc_batch: plan once (graph capture), run many times."""
 

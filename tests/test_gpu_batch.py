@@ -82,3 +82,38 @@ def test_autotuned_batch_equals_separate_searches(gpu):
     b = B.Batch(inst, tuning)
     b.run()
     assert _batch_hits(b) == want
+
+
+def test_multibeam_regions_in_place(gpu):
+    # The beamformer feed (beamforming_pipeline.cpp:230-245): every (beam,
+    # coarse channel) region of a multibeam buffer searched in place through
+    # its row stride equals searching a contiguous copy of the region
+    # (MultibeamBuffer::copyRegionAsync, multibeam_buffer.cu:65-76).
+    import torch
+    beams, T, ccs, ncc = 3, 16, 1 << 15, 4
+    multibeam = torch.empty((beams, T, ncc * ccs), dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(8)
+    for b in range(beams):
+        tones = [(float(rng.uniform(0, ncc * ccs)), float(rng.uniform(-1, 1)), 4.0)
+                 for _ in range(10)]
+        sc.gen_waterfall(multibeam[b], T, seed=70 + b, tones=tones)
+    meta = sc.FilterbankMetadata(fch1=8000.0, foff=FOFF, coarse_channel_size=ccs,
+                                 source_names=["b%d" % b for b in range(beams)])
+    inst = B.multibeam_instances(multibeam, T, ccs, FOFF, TSAMP, 2.0, 0.0, 8.0, metadata=meta)
+    assert len(inst) == beams * ncc and inst[5].row_stride == ncc * ccs
+    b = B.Batch(inst)
+    b.run()
+    dd = sc.Dedopplerer(T, ccs, FOFF, TSAMP)
+    dd.set_stats_mode(sc.SC_STATS_DEVICE)
+    total = 0
+    for i, it in enumerate(inst):
+        beam, cc = divmod(i, ncc)
+        region = multibeam[beam, :, cc * ccs:(cc + 1) * ccs].contiguous()
+        want = []
+        dd.search(sc.FilterbankBuffer(T, ccs, data=region), meta, beam, cc, 2.0, 0.0, 8.0, want)
+        got = b.hits(i)
+        assert [h.key() + (h.snr, h.frequency) for h in got] == \
+            [h.key() + (h.snr, h.frequency) for h in want]
+        assert all(h.beam == beam for h in got)
+        total += len(got)
+    assert total > 0
