@@ -195,7 +195,8 @@ def main():
         from seticore_amd import batch as B
         tuning = []
         if args.lanes:
-            tuning = [B.ScTuning(T, N, args.lanes, args.group, args.prefetch, 0 if T == 32 else 4, 0)]
+            tuning = [B.ScTuning(T, N, args.lanes, args.group, args.prefetch, 4 if T <= 16 else 0,
+                                 0)]
         batch = B.Batch([B.instance(data[cc], T, FOFF, TSAMP, MAX_DRIFT, 0.0, SNR_THRESHOLD,
                                     beam=sc.NO_BEAM, coarse_channel=cc, metadata=meta)
                          for cc in range(num_cc)], tuning)
@@ -260,8 +261,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic() if (T, MAX_DRIFT) == (16, 10.0) else None,
-            "kernel": "fused_tree_top_kernel" if T <= 16 else
-                      "column sums + two-pass tree (level16_kernel, upper_tree_kernel)",
+            "kernel": "fused_tree_top_kernel" if T <= 16 else "fused32_kernel" if T == 32 else
+                      "column sums + staged_tree_kernel",
             "kernel_ms_per_launch": round(avg_ms, 4),
             "alg_bytes_per_launch": alg_bytes,
         },

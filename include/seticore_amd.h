@@ -88,10 +88,12 @@ sc_status sc_dd_memory_usage(const sc_dd* dd, size_t* bytes);
 /* Selects sc_stats_mode for later searches (no counterpart; default host-exact). */
 sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
 
-/* Shape of the one-pass tree kernel used when rounded_timesteps <= 32
- * (at 32 `prefetch` is the number of waves per workgroup sharing the window:
- * (C, G, waves) in {(4,4,0), (2,2,0), (2,4,0), (2,4,2), (2,8,2), (2,8,4),
- * (4,8,2)}, 0 meaning one; selecting it undoes sc_dd_set_two_pass):
+/* Shape of the one-pass tree kernel used when rounded_timesteps <= 256 (the
+ * default; selecting a shape undoes sc_dd_set_two_pass). At 64..256 (level-16
+ * trees staged in LDS) the shape is (4, drift blocks per workgroup in
+ * {1, 2, 4, 8, 16, 32}, 0). At 32 `prefetch` is the number of waves per
+ * workgroup sharing the window: (C, G, waves) in {(4,4,0), (2,2,0), (2,4,0),
+ * (2,4,2), (2,8,2), (2,8,4), (4,8,2)}, 0 meaning one. At 16 and below:
  * channels held per lane (4 or 8), drift blocks per LDS window (a multiple
  * of channels_per_lane) and the window mode in `prefetch` (0: one LDS window
  * per workgroup; 1: each workgroup walks several windows, prefetching the
@@ -102,8 +104,8 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
  * used by the autotuner. */
 sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, int prefetch);
 
-/* Shape of the two-pass search used when rounded_timesteps is 64..256, and at
- * 32 once this is called (the default there is the one-pass kernel) (the
+/* Selects the two-pass search (level-16 trees through an HBM scratch array)
+ * for rounded_timesteps 32..256 instead of the one-pass kernels (the
  * reference's tiledTaylorTree<32> / twoStageTaylorTree range, taylor.cu:261-350):
  * drift blocks per LDS window in the level-16 pass (4 or 8) and drift blocks
  * per launch pair (1..64; 0 = as many as keep the scratch within 4.5 GB, at

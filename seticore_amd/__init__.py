@@ -355,7 +355,7 @@ class Dedopplerer:
                                      int(prefetch)))
 
     def set_two_pass(self, drift_group, blocks_per_launch=0):
-        """Shape of the T_r = 32..256 two-pass search (speed only)."""
+        """Selects the T_r = 32..256 two-pass search and its shape (speed only)."""
         check(lib().sc_dd_set_two_pass(self._h, int(drift_group), int(blocks_per_launch)))
 
     def set_graphs(self, enable):

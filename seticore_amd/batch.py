@@ -139,6 +139,7 @@ class Batch:
 FUSED_CANDIDATES = [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1), (4, 4, 2),
                     (4, 8, 2), (4, 16, 2), (8, 8, 0), (8, 16, 0), (8, 4, 0)]
 FUSED32_CANDIDATES = [(4, 4, 0), (2, 2, 0), (2, 4, 0), (2, 4, 2), (2, 8, 2), (2, 8, 4), (4, 8, 2)]
+STAGED_CANDIDATES = [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 32, 0)]
 TWO_PASS_CANDIDATES = [(4, 1), (4, 2), (4, 4), (8, 4), (8, 8)]
 
 _cache = {}
@@ -178,8 +179,9 @@ def autotune(T_r, N, foff, tsamp, max_drift, reps=3, cache_file=None, log=None):
     dd.set_stats_mode(SC_STATS_DEVICE)
     dd.set_graphs(False)
     best = None
-    if T_r <= 32:
-        for c, g, p in (FUSED_CANDIDATES if T_r <= 16 else FUSED32_CANDIDATES):
+    if T_r <= 256:
+        for c, g, p in (FUSED_CANDIDATES if T_r <= 16 else
+                        FUSED32_CANDIDATES if T_r == 32 else STAGED_CANDIDATES):
             try:
                 dd.set_tuning(c, g, p)
             except SeticoreError:

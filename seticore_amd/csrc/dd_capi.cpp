@@ -64,8 +64,10 @@ void default_tuning(int T, int* lanes, int* group, int* prefetch) {
   // Measured on MI355X: at 16 x 2^20 (132 drift blocks) C=4, G=4 with chunked
   // windows (mode 2) is the fastest T=16 shape; at 32 x 2^20 (66 blocks) the
   // one-pass kernel with C=2, G=4 and two waves per workgroup (2.26 ms, vs
-  // 4.25 ms for the two-pass search).
-  *lanes = T == 16 ? 4 : T == 32 ? 2 : 8;
+  // 4.25 ms for the two-pass search); at 64..256 x 2^20 (66 blocks) the
+  // LDS-staged one-pass kernel with 16 drift blocks per workgroup (2.96,
+  // 8.89, 28.8 ms vs 7.83, 16.5, 33.7 ms for the two-pass search).
+  *lanes = T == 16 ? 4 : T == 32 ? 2 : T <= 256 ? 4 : 8;
   *group = (T == 16 || T == 32) ? 4 : 16;
   *prefetch = (T == 16 || T == 32) ? 2 : 0;
 }
@@ -440,11 +442,10 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
   if (s == SC_OK)
     s = host_alloc((void**)&dd->h_cand, dd->cand_capacity * sizeof(int4), "candidates");
   if (s == SC_OK) s = host_alloc((void**)&dd->h_stats, 4 * sizeof(float), "stats");
-  if (rounded == 32) {
-    // The one-pass kernel needs no scratch; sc_dd_set_two_pass allocates it.
-  } else if (sc::two_pass_supported(rounded, dd->pass_group)) {
+  if (sc::two_pass_supported(rounded, dd->pass_group)) {
+    // The one-pass kernels (default up to 256 time steps) need no scratch;
+    // sc_dd_set_two_pass allocates the two-pass search's.
     dd->pass_pitch = sc::two_pass_row_pitch(rounded, num_channels);
-    if (s == SC_OK) s = ensure_pass_scratch(dd, pass_blocks(dd));
   } else if (s == SC_OK && rounded > 16) {
     s = ensure_tree_scratch(dd);
   }
@@ -494,7 +495,7 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode) {
 
 sc_status sc_dd_set_tuning(sc_dd* dd, int channels_per_lane, int drift_group, int prefetch) {
   if (!dd) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
-  if (dd->rounded > 32) return fail(SC_ERR_UNSUPPORTED, "no one-pass kernel above 32 time steps");
+  if (dd->rounded > 256) return fail(SC_ERR_UNSUPPORTED, "no one-pass kernel above 256 time steps");
   if (!sc::fused_supported(dd->rounded, channels_per_lane, drift_group, prefetch))
     return fail(SC_ERR_UNSUPPORTED, "no fused kernel for this (lanes, group, prefetch) shape");
   dd->lanes = channels_per_lane;
@@ -544,7 +545,7 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
   // dedoppler.cu:205 (the reference zeroes the float sums; the key 0 means "no
   // positive path"), plus the statistics histograms.
   SC_HIP(sc::launch_reset(dd->top_key, N, dd->stats_hist, stream), "reset");
-  const bool one_pass = (T <= 16 || (T == 32 && !dd->use_two_pass)) &&
+  const bool one_pass = (T <= 16 || !dd->use_two_pass) &&
                         sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch);
   if (overlap_stats) {
     SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
@@ -1179,7 +1180,7 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
         for (int k = 0; k < num_tuning; ++k) {
           const sc_tuning& t = tuning[k];
           if (t.rounded_timesteps != T || t.num_channels != in.num_channels) continue;
-          if (T <= 16 || (T == 32 && t.pass_group == 0))
+          if (T <= 16 || t.pass_group == 0)
             s = sc_dd_set_tuning(dd, t.channels_per_lane, t.drift_group, t.prefetch);
           else if (sc::two_pass_supported(T, t.pass_group))
             s = sc_dd_set_two_pass(dd, t.pass_group, t.pass_blocks);

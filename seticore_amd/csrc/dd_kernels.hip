@@ -767,6 +767,266 @@ __global__ void __launch_bounds__(kWave * kUpperWaves)
   }
 }
 
+// ---------------------------------------------------------------------------
+// T_r = 64..256 in one pass: the level-16 trees staged in LDS by groups of
+// offsets.
+//
+// With the decomposition above, a workgroup takes one channel tile and, in
+// turn, each drift block of its chunk and each group of QG = 64/M consecutive
+// level-16 offsets q0..q0+QG-1: phase 1 computes R_q[k] (q in the group, every
+// time block k: QG*M = 64 rows) into LDS, phase 2 runs the M-row trees of
+// those offsets from LDS and folds the keys. Nothing goes through HBM but the
+// input. Offsets of a group differ by at most QG-1, so time block k's trees
+// are needed over only Wr + (QG-1)*k channels, which one wave's 240 valid
+// level-16 outputs cover for every k. Offset q of level 16 needs offset q >> 1
+// of level 8, q >> 2 of level 4 and q >> 3 of level 2, so a group's partial
+// tree takes at most 5 adds per output (4 for the whole tree), in exactly the
+// whole tree's order.
+template <int M>
+struct Staged {
+  static constexpr int C = 4;
+  static constexpr int T = 16 * M;
+  static constexpr int QG = 64 / M;    // level-16 offsets per group
+  static constexpr int kGroups = 16 / QG;
+  static constexpr int kWaves = 4;
+  static constexpr int kLevel16Out = TileShape<16, C>::kOut;   // 240 channels
+  static constexpr int kLevel16Lanes = TileShape<16, C>::kValidLanes;
+  static constexpr int kWr = kLevel16Out - (QG - 1) * (M - 1);  // R columns valid for every k
+  static constexpr int kOut = (kWr - (M - 1)) / C * C;          // output channels per tile
+  // LDS row of R: kWave * C columns stored de-interleaved by C (column s at
+  // sub-row s % C, position s / C), so that phase 1's shifted writes and
+  // phase 2's reads are both one word per lane at consecutive addresses.
+  static constexpr int kSub = kWave;
+  static constexpr int kRow = C * kSub;
+  static constexpr int kFloats = QG * M * kRow;  // 64 KiB
+  static_assert(kOut <= TileShape<M, C>::kOut, "phase 2 tile");
+  static_assert(QG * M == 64 && kGroups * QG == 16, "group shape");
+};
+
+// Stage L of the 16-row tree restricted to output offsets [LO, HI] of every
+// time sub-block: entry tb*L + p = entry tb*L + p/2 + (entry tb*L + L/2 + p/2
+// shifted by (p+1)/2 channels), as tree_stage computes it.
+template <int C, int L, int LO, int HI>
+__device__ __forceinline__ void tree_stage_range(float (&v)[16][C]) {
+  float w[16][C];
+#pragma unroll
+  for (int tb = 0; tb < 16 / L; ++tb) {
+#pragma unroll
+    for (int q = LO / 2; q <= HI / 2; ++q) {
+      const int ra = tb * L + q;
+      const int rb = tb * L + L / 2 + q;
+      float bx[C + L / 2];
+#pragma unroll
+      for (int j = 0; j < C; ++j) bx[j] = v[rb][j];
+#pragma unroll
+      for (int k = 0; k < L / 2; ++k) {
+        if (k <= q) bx[C + k] = from_next_lane(bx[k]);
+      }
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        if (2 * q >= LO && 2 * q <= HI) w[tb * L + 2 * q][j] = v[ra][j] + bx[j + q];
+        if (2 * q + 1 >= LO && 2 * q + 1 <= HI) w[tb * L + 2 * q + 1][j] = v[ra][j] + bx[j + q + 1];
+      }
+    }
+  }
+#pragma unroll
+  for (int tb = 0; tb < 16 / L; ++tb)
+#pragma unroll
+    for (int p = LO; p <= HI; ++p)
+#pragma unroll
+      for (int j = 0; j < C; ++j) v[tb * L + p][j] = w[tb * L + p][j];
+}
+
+// Level-16 offsets [Q0, Q1] (rows Q0..Q1 of v on return).
+template <int C, int Q0, int Q1, int L = 2>
+__device__ __forceinline__ void tree_range(float (&v)[16][C]) {
+  if constexpr (L <= 16) {
+    constexpr int sh = L == 2 ? 3 : L == 4 ? 2 : L == 8 ? 1 : 0;
+    tree_stage_range<C, L, (Q0 >> sh), (Q1 >> sh)>(v);
+    tree_range<C, Q0, Q1, L * 2>(v);
+  }
+}
+
+// Phase 1 input for time block k: the 16 rows 16k.. at start channels
+// base + 4*lane + j (row t shifted by t*b, the block's own coordinates).
+template <int M, typename In>
+__device__ __forceinline__ void staged_load(const In* __restrict__ x, long stride, int N, int k,
+                                            int b, long base, float (&v)[16][Staged<M>::C]) {
+  constexpr int C = Staged<M>::C;
+  const int lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const In* row = x + (long)(16 * k + t) * stride;
+    const long g0 = base + (long)t * b;
+    if (g0 >= 0 && g0 + kWave * C <= N) {
+      const In* src = row + g0 + lane * C;
+      if constexpr (sizeof(In) == 4) {
+        typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+        const f4u w = *(const f4u*)src;
+        v[t][0] = w.x;
+        v[t][1] = w.y;
+        v[t][2] = w.z;
+        v[t][3] = w.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < C; ++j) v[t][j] = to_f32<In>(src[j]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        const long gc = g0 + lane * C + j;
+        const bool ok = gc >= 0 && gc < N;
+        const float val = to_f32<In>(row[ok ? gc : 0]);
+        v[t][j] = ok ? val : 0.0f;
+      }
+    }
+  }
+}
+
+// Phase 1 for time block k: the partial tree of the loaded rows, and
+// R_{q0+qq}[k][s] for s = 4*lane + j - k*qq into LDS row qq*M + k.
+template <int M, int G_IDX>
+__device__ __forceinline__ void staged_phase1(float (&v)[16][Staged<M>::C], int k,
+                                              lds_float* R) {
+  using St = Staged<M>;
+  constexpr int C = St::C;
+  constexpr int Q0 = G_IDX * St::QG;
+  const int lane = threadIdx.x % kWave;
+  tree_range<C, Q0, Q0 + St::QG - 1>(v);
+  if (lane < St::kLevel16Lanes) {
+#pragma unroll
+    for (int qq = 0; qq < St::QG; ++qq) {
+      // s = 4*(lane - m) + (j - e) with d = k*qq = 4m + e: column j - e of
+      // position lane - m, or column j - e + 4 of position lane - m - 1.
+      const int d = k * qq;
+      const int m = d >> 2;
+      const int e = d & 3;
+      lds_float* row = R + (qq * M + k) * St::kRow;
+      const int pos = lane - m;
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        const int jj = j - e;
+        const int col = jj >= 0 ? jj : jj + C;
+        const int p = jj >= 0 ? pos : pos - 1;
+        if (p >= 0) row[col * St::kSub + p] = v[Q0 + qq][j];
+      }
+    }
+  }
+}
+
+// Phase 2 for one offset q = q0 + qq: R_q[0..M-1] from LDS, the M-row tree at
+// drift 0, masking (dedoppler.cu:40-46) and the keys.
+template <int M>
+__device__ __forceinline__ void staged_phase2(const lds_float* R, int qq, int q, int b, int c_w,
+                                              int N, bool interior, unsigned low_b,
+                                              double (&best)[Staged<M>::C]) {
+  using St = Staged<M>;
+  constexpr int C = St::C;
+  constexpr int T = St::T;
+  const int lane = threadIdx.x % kWave;
+  float v[M][C];
+  const lds_float* src = R + qq * M * St::kRow + lane;
+#pragma unroll
+  for (int k = 0; k < M; ++k)
+#pragma unroll
+    for (int j = 0; j < C; ++j) v[k][j] = src[k * St::kRow + j * St::kSub];
+  tree_regs<M, C>(v);
+  if (!interior) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int c = c_w + lane * C + j;
+      const int last0 = c + (T - 1) * b;
+      const int pmax = (c >= N || last0 < 0) ? 0 : min(T, N - last0);
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+        if (q * M + r >= pmax) v[r][j] = 0.0f;
+    }
+  }
+  const unsigned low0 = low_b - (unsigned)(q * M);
+#pragma unroll
+  for (int r = 0; r < M; ++r) {
+    const unsigned low = low0 - (unsigned)r;
+#pragma unroll
+    for (int j = 0; j < C; ++j) best[j] = fmax(best[j], make_key(v[r][j], low));
+  }
+}
+
+template <int M, int G_IDX, typename In>
+__device__ __forceinline__ void staged_groups(const In* __restrict__ x, long stride, int N, int b,
+                                              int c_w, bool interior, unsigned low_b,
+                                              lds_float* R, double (&best)[Staged<M>::C]) {
+  using St = Staged<M>;
+  if constexpr (G_IDX < St::kGroups) {
+    constexpr int Q0 = G_IDX * St::QG;
+    const int wave = threadIdx.x / kWave;
+    // (Double-buffering the rows of the next time block measured slower: the
+    // extra registers halve the waves per SIMD.)
+    for (int k = wave; k < M; k += St::kWaves) {
+      float v[16][St::C];
+      staged_load<M, In>(x, stride, N, k, b, (long)c_w + (long)k * (16 * b + Q0), v);
+      staged_phase1<M, G_IDX>(v, k, R);
+    }
+    __syncthreads();
+    for (int qq = wave; qq < St::QG; qq += St::kWaves)
+      staged_phase2<M>(R, qq, Q0 + qq, b, c_w, N, interior, low_b, best);
+    __syncthreads();
+    staged_groups<M, G_IDX + 1, In>(x, stride, N, b, c_w, interior, low_b, R, best);
+  }
+}
+
+// Grid: one workgroup of 4 waves per (channel tile, chunk of G consecutive
+// drift blocks; G at run time), tile-major and dealt so that each XCD takes a contiguous
+// range of items. The waves keep their own key maxima over the chunk and merge
+// them through LDS before one atomic max per channel.
+template <int M, typename In>
+__global__ void __launch_bounds__(kWave * Staged<M>::kWaves)
+    staged_tree_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
+                       int G, int nchunks, int total_items, top_key_t* __restrict__ top_key) {
+  using St = Staged<M>;
+  constexpr int C = St::C;
+  constexpr int T = St::T;
+  __shared__ __attribute__((aligned(16))) float r_storage[St::kFloats];
+  lds_float* R = (lds_float*)r_storage;
+  const int per_xcd = gridDim.x / 8;
+  const int item = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (item >= total_items) return;
+  const int tile = item / nchunks;
+  const int chunk = item % nchunks;
+  const int c_w = tile * St::kOut;
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  double best[C];
+#pragma unroll
+  for (int j = 0; j < C; ++j) best[j] = __longlong_as_double((long long)kEmptyKey);
+  const bool tile_in = c_w + St::kOut - 1 < N;
+  const int g_end = min(nblocks, (chunk + 1) * G);
+  for (int g = chunk * G; g < g_end; ++g) {
+    const int b = bmin + g;
+    const bool interior =
+        tile_in && (c_w + (T - 1) * b >= 0) && (c_w + St::kOut - 1 + (T - 1) * b + (T - 1) < N);
+    const unsigned low_b = (unsigned)kCodeMask - (unsigned)(g * T);
+    staged_groups<M, 0, In>(x, stride, N, b, c_w, interior, low_b, R, best);
+  }
+  // The last phase 2 ended with a barrier: R is free for the merge.
+  double* red = (double*)r_storage;
+  if (wave > 0) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) red[(wave - 1) * kWave * C + lane * C + j] = best[j];
+  }
+  __syncthreads();
+  if (wave == 0 && lane * C < St::kOut) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+#pragma unroll
+      for (int w = 0; w < St::kWaves - 1; ++w)
+        best[j] = fmax(best[j], red[w * kWave * C + lane * C + j]);
+      const int c = c_w + lane * C + j;
+      const unsigned long long key = (unsigned long long)__double_as_longlong(best[j]);
+      if (c < N && key > kEmptyKey) atomicMax(&top_key[c], key);
+    }
+  }
+}
+
 // taylor.cu:97-109 + taylor.h:30-86: one stage in global memory.
 __global__ void one_step_kernel(const float* __restrict__ src, float* __restrict__ dst, int T,
                                 int N, int L, int drift) {
@@ -1515,6 +1775,32 @@ hipError_t two_pass_dispatch(const In* x, long stride, int T, int N, int bmin, i
   return hipErrorInvalidValue;
 }
 
+template <int M, typename In>
+hipError_t launch_staged_typed(const In* x, long stride, int N, int bmin, int nblocks, int G,
+                               top_key_t* top_key, hipStream_t stream) {
+  using St = Staged<M>;
+  const long ntiles = ((long)N + St::kOut - 1) / St::kOut;
+  const long nchunks = ((long)nblocks + G - 1) / G;
+  const long total = ntiles * nchunks;
+  if (total > (1L << 30)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((staged_tree_kernel<M, In>), dim3((int)((total + 7) / 8 * 8)),
+                     dim3(kWave * St::kWaves), 0, stream, x, stride, N, bmin, nblocks, G,
+                     (int)nchunks, (int)total, top_key);
+  return hipGetLastError();
+}
+
+template <typename In>
+hipError_t staged_dispatch(const In* x, long stride, int T, int N, int bmin, int nblocks, int G,
+                           top_key_t* top_key, hipStream_t stream) {
+  switch (T) {
+    case 64: return launch_staged_typed<4, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
+    case 128: return launch_staged_typed<8, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
+    case 256:
+      return launch_staged_typed<16, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
 template <int T, typename In>
 hipError_t tree_blocks_typed(const In* x, long stride, int Tall, int N, int drift, float* out,
                              hipStream_t stream) {
@@ -1543,6 +1829,8 @@ hipError_t tree_blocks_dispatch(const In* x, long stride, int T, int N, int drif
 }  // namespace
 
 bool fused_supported(int T, int C, int G, int P) {
+  if (T == 64 || T == 128 || T == 256)
+    return C == 4 && P == 0 && (G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32);
   if (T == 32)
     return (P == 0 && ((C == 4 && G == 4) || (C == 2 && (G == 2 || G == 4)))) ||
            (P == 2 && ((C == 2 && (G == 4 || G == 8)) || (C == 4 && G == 8))) ||
@@ -1558,6 +1846,25 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
                                  int nblocks, int C, int G, int P, top_key_t* top_key,
                                  float* colsum, hipStream_t stream, const int* tile_list,
                                  const int* tile_count) {
+  if (T == 64 || T == 128 || T == 256) {
+    if (tile_list || !fused_supported(T, C, G, P)) return hipErrorInvalidValue;
+    if (colsum) {
+      const hipError_t e = launch_colsum(x, dtype, stride, T, N, colsum, stream);
+      if (e != hipSuccess) return e;
+    }
+    switch (dtype) {
+      case 0:
+        return staged_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, G, top_key,
+                                      stream);
+      case 1:
+        return staged_dispatch<__half>((const __half*)x, stride, T, N, bmin, nblocks, G,
+                                       top_key, stream);
+      case 2:
+        return staged_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N, bmin,
+                                               nblocks, G, top_key, stream);
+    }
+    return hipErrorInvalidValue;
+  }
   if (T == 32) {
     if (tile_list) return hipErrorInvalidValue;
     switch (dtype) {
