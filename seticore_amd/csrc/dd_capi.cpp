@@ -65,10 +65,10 @@ void default_tuning(int T, int* lanes, int* group, int* prefetch) {
   // windows (mode 2) is the fastest T=16 shape; at 32 x 2^20 (66 blocks) the
   // one-pass kernel with C=2, G=4 and two waves per workgroup (2.26 ms, vs
   // 4.25 ms for the two-pass search); at 64..256 x 2^20 (66 blocks) the
-  // LDS-staged one-pass kernel with 16 drift blocks per workgroup (2.96,
-  // 8.89, 28.8 ms vs 7.83, 16.5, 33.7 ms for the two-pass search).
+  // LDS-staged one-pass kernel with 16, 8, 8 drift blocks per workgroup
+  // (3.02, 6.85, 24.3 ms vs 7.83, 16.5, 33.7 ms for the two-pass search).
   *lanes = T == 16 ? 4 : T == 32 ? 2 : T <= 256 ? 4 : 8;
-  *group = (T == 16 || T == 32) ? 4 : 16;
+  *group = (T == 16 || T == 32) ? 4 : (T == 128 || T == 256) ? 8 : 16;
   *prefetch = (T == 16 || T == 32) ? 2 : 0;
 }
 

@@ -793,13 +793,23 @@ struct Staged {
   static constexpr int kLevel16Lanes = TileShape<16, C>::kValidLanes;
   static constexpr int kWr = kLevel16Out - (QG - 1) * (M - 1);  // R columns valid for every k
   static constexpr int kOut = (kWr - (M - 1)) / C * C;          // output channels per tile
-  // LDS row of R: kWave * C columns stored de-interleaved by C (column s at
-  // sub-row s % C, position s / C), so that phase 1's shifted writes and
-  // phase 2's reads are both one word per lane at consecutive addresses.
-  static constexpr int kSub = kWave;
+  // LDS row of R: columns stored de-interleaved by C (column s at sub-row
+  // s % C, position s / C), so that phase 1's shifted writes and phase 2's
+  // reads are both one word per lane at consecutive addresses.
+  // Only positions (lanes) below kUsed feed a valid output (the tile's
+  // outputs and the M-1 channels after them), so a sub-row holds kSub >= kUsed
+  // positions; lanes beyond read into the next sub-row (or the pad after the
+  // last row) and produce discarded outputs. 52 KiB, three workgroups per CU.
+  static constexpr int kUsed = (kOut + M - 1 + C - 1) / C;
+  static constexpr int kSub = (kUsed + 3) / 4 * 4;
   static constexpr int kRow = C * kSub;
-  static constexpr int kFloats = QG * M * kRow;  // 64 KiB
+  static constexpr int kFloats = QG * M * kRow + kWave;
+  // Waves per SIMD the register allocation must allow: LDS admits three
+  // workgroups (12 waves) per CU. Measured: 168 VGPRs (one spilled) at M = 16
+  // beat 197 at two waves; M = 4 keeps its 237 registers.
+  static constexpr int kMinWaves = M == 4 ? 2 : 3;
   static_assert(kOut <= TileShape<M, C>::kOut, "phase 2 tile");
+  static_assert(kUsed <= kLevel16Lanes, "phase 1 writes every position read");
   static_assert(QG * M == 64 && kGroups * QG == 16, "group shape");
 };
 
@@ -908,7 +918,7 @@ __device__ __forceinline__ void staged_phase1(float (&v)[16][Staged<M>::C], int 
         const int jj = j - e;
         const int col = jj >= 0 ? jj : jj + C;
         const int p = jj >= 0 ? pos : pos - 1;
-        if (p >= 0) row[col * St::kSub + p] = v[Q0 + qq][j];
+        if (p >= 0 && p < St::kSub) row[col * St::kSub + p] = v[Q0 + qq][j];
       }
     }
   }
@@ -979,7 +989,7 @@ __device__ __forceinline__ void staged_groups(const In* __restrict__ x, long str
 // range of items. The waves keep their own key maxima over the chunk and merge
 // them through LDS before one atomic max per channel.
 template <int M, typename In>
-__global__ void __launch_bounds__(kWave * Staged<M>::kWaves)
+__global__ void __launch_bounds__(kWave * Staged<M>::kWaves, Staged<M>::kMinWaves)
     staged_tree_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
                        int G, int nchunks, int total_items, top_key_t* __restrict__ top_key) {
   using St = Staged<M>;
