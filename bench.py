@@ -261,8 +261,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic() if (T, MAX_DRIFT) == (16, 10.0) else None,
-            "kernel": "fused_tree_top_kernel" if T <= 16 else "fused32_kernel" if T == 32 else
-                      "column sums + staged_tree_kernel",
+            "kernel": "fused_tree_top_kernel" if T <= 16 else "column sums + staged_tree_kernel",
             "kernel_ms_per_launch": round(avg_ms, 4),
             "alg_bytes_per_launch": alg_bytes,
         },

@@ -91,9 +91,10 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
 /* Shape of the one-pass tree kernel used when rounded_timesteps <= 256 (the
  * default; selecting a shape undoes sc_dd_set_two_pass). At 64..256 (level-16
  * trees staged in LDS) the shape is (4, drift blocks per workgroup in
- * {1, 2, 4, 8, 16, 32}, 0). At 32 `prefetch` is the number of waves per
- * workgroup sharing the window: (C, G, waves) in {(4,4,0), (2,2,0), (2,4,0),
- * (2,4,2), (2,8,2), (2,8,4), (4,8,2)}, 0 meaning one. At 16 and below:
+ * {1, 2, 4, 8, 16, 32}, 0). At 32 the same kernel is (4, G, 8) (the default);
+ * otherwise `prefetch` is the number of waves per workgroup sharing the
+ * window of the register-parked kernel: (C, G, waves) in {(4,4,0), (2,2,0),
+ * (2,4,0), (2,4,2), (2,8,2), (2,8,4), (4,8,2)}, 0 meaning one. At 16 and below:
  * channels held per lane (4 or 8), drift blocks per LDS window (a multiple
  * of channels_per_lane) and the window mode in `prefetch` (0: one LDS window
  * per workgroup; 1: each workgroup walks several windows, prefetching the

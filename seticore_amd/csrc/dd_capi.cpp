@@ -62,14 +62,15 @@ size_t dtype_size(int dtype) { return dtype == SC_F32 ? 4 : 2; }
 // LDS window.
 void default_tuning(int T, int* lanes, int* group, int* prefetch) {
   // Measured on MI355X: at 16 x 2^20 (132 drift blocks) C=4, G=4 with chunked
-  // windows (mode 2) is the fastest T=16 shape; at 32 x 2^20 (66 blocks) the
-  // one-pass kernel with C=2, G=4 and two waves per workgroup (2.26 ms, vs
-  // 4.25 ms for the two-pass search); at 64..256 x 2^20 (66 blocks) the
-  // LDS-staged one-pass kernel with 16, 8, 8 drift blocks per workgroup
-  // (3.02, 6.85, 24.3 ms vs 7.83, 16.5, 33.7 ms for the two-pass search).
-  *lanes = T == 16 ? 4 : T == 32 ? 2 : T <= 256 ? 4 : 8;
-  *group = (T == 16 || T == 32) ? 4 : (T == 128 || T == 256) ? 8 : 16;
-  *prefetch = (T == 16 || T == 32) ? 2 : 0;
+  // windows (mode 2) is the fastest T=16 shape; at 32..256 x 2^20 (66 blocks)
+  // the LDS-staged one-pass kernel with 8, 16, 8, 8 drift blocks per
+  // workgroup (1.29, 3.02, 6.85, 24.3 ms against 2.25 ms for fused32_kernel at
+  // 32 and 4.25, 7.83, 16.5, 33.7 ms for the two-pass search).
+  // Below 16 time steps: 8 channels per lane, 16 drift blocks per window.
+  const bool staged = T >= 32 && T <= 256;
+  *lanes = (T == 16 || staged) ? 4 : 8;
+  *group = T == 16 ? 4 : T == 64 ? 16 : staged ? 8 : 16;
+  *prefetch = T == 16 ? 2 : T == 32 ? sc::kStaged32 : 0;
 }
 
 }  // namespace
@@ -416,6 +417,12 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
   // dedoppler.cu:87
   dd->resolution = 1e6 * foff_mhz / (dd->drift_timesteps * tsamp_s);
   default_tuning(rounded, &dd->lanes, &dd->group, &dd->prefetch);
+  if (rounded <= 256 && !sc::fused_supported(rounded, dd->lanes, dd->group, dd->prefetch)) {
+    // Every T_r up to 256 has a compiled one-pass default; never fall back
+    // to the staged global-memory path silently.
+    delete dd;
+    return fail(SC_ERR_UNSUPPORTED, "no compiled default kernel shape for this T_r");
+  }
   hipGetDevice(&dd->device);
 
   const size_t N = num_channels;

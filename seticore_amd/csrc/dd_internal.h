@@ -29,7 +29,10 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
                                  const int* tile_count = nullptr);
 int fused_tile_width(int T, int C);
 
-// Whether launch_fused_tree_top has a kernel for (T, C, G, P).
+// Whether launch_fused_tree_top has a kernel for (T, C, G, P). At T = 32 the
+// `prefetch` value kStaged32 selects the LDS-staged kernel of T = 64..256
+// (shape (4, G, kStaged32)) instead of fused32_kernel.
+constexpr int kStaged32 = 8;
 bool fused_supported(int T, int C, int G, int P);
 
 // T_r = 32..256: path sums of drift blocks [b_first, b_first + nb) folded into

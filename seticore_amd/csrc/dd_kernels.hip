@@ -786,9 +786,9 @@ template <int M>
 struct Staged {
   static constexpr int C = 4;
   static constexpr int T = 16 * M;
-  static constexpr int QG = 64 / M;    // level-16 offsets per group
+  static constexpr int QG = M == 2 ? 16 : 64 / M;  // level-16 offsets per group
   static constexpr int kGroups = 16 / QG;
-  static constexpr int kWaves = 4;
+  static constexpr int kWaves = M == 2 ? 2 : 4;
   static constexpr int kLevel16Out = TileShape<16, C>::kOut;   // 240 channels
   static constexpr int kLevel16Lanes = TileShape<16, C>::kValidLanes;
   static constexpr int kWr = kLevel16Out - (QG - 1) * (M - 1);  // R columns valid for every k
@@ -799,7 +799,8 @@ struct Staged {
   // Only positions (lanes) below kUsed feed a valid output (the tile's
   // outputs and the M-1 channels after them), so a sub-row holds kSub >= kUsed
   // positions; lanes beyond read into the next sub-row (or the pad after the
-  // last row) and produce discarded outputs. 52 KiB, three workgroups per CU.
+  // last row) and produce discarded outputs. 52 KiB, three workgroups per CU
+  // (M = 2: 32 rows, 29 KiB).
   static constexpr int kUsed = (kOut + M - 1 + C - 1) / C;
   static constexpr int kSub = (kUsed + 3) / 4 * 4;
   static constexpr int kRow = C * kSub;
@@ -807,10 +808,11 @@ struct Staged {
   // Waves per SIMD the register allocation must allow: LDS admits three
   // workgroups (12 waves) per CU. Measured: 168 VGPRs (one spilled) at M = 16
   // beat 197 at two waves; M = 4 keeps its 237 registers.
-  static constexpr int kMinWaves = M == 4 ? 2 : 3;
+  static constexpr int kMinWaves = M <= 4 ? 2 : 3;
   static_assert(kOut <= TileShape<M, C>::kOut, "phase 2 tile");
   static_assert(kUsed <= kLevel16Lanes, "phase 1 writes every position read");
-  static_assert(QG * M == 64 && kGroups * QG == 16, "group shape");
+  static_assert(QG * M <= 64 && kGroups * QG == 16, "group shape");
+  static_assert(3 * kWave * C * 2 <= kFloats, "key merge buffer");
 };
 
 // Stage L of the 16-row tree restricted to output offsets [LO, HI] of every
@@ -1803,6 +1805,7 @@ template <typename In>
 hipError_t staged_dispatch(const In* x, long stride, int T, int N, int bmin, int nblocks, int G,
                            top_key_t* top_key, hipStream_t stream) {
   switch (T) {
+    case 32: return launch_staged_typed<2, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
     case 64: return launch_staged_typed<4, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
     case 128: return launch_staged_typed<8, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
     case 256:
@@ -1838,9 +1841,15 @@ hipError_t tree_blocks_dispatch(const In* x, long stride, int T, int N, int drif
 
 }  // namespace
 
+bool staged_shape(int T, int C, int G, int P) {
+  const bool g_ok = G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32;
+  if (T == 64 || T == 128 || T == 256) return C == 4 && P == 0 && g_ok;
+  return T == 32 && C == 4 && P == kStaged32 && g_ok;
+}
+
 bool fused_supported(int T, int C, int G, int P) {
-  if (T == 64 || T == 128 || T == 256)
-    return C == 4 && P == 0 && (G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32);
+  if (T == 64 || T == 128 || T == 256 || (T == 32 && P == kStaged32))
+    return staged_shape(T, C, G, P);
   if (T == 32)
     return (P == 0 && ((C == 4 && G == 4) || (C == 2 && (G == 2 || G == 4)))) ||
            (P == 2 && ((C == 2 && (G == 4 || G == 8)) || (C == 4 && G == 8))) ||
@@ -1856,8 +1865,8 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
                                  int nblocks, int C, int G, int P, top_key_t* top_key,
                                  float* colsum, hipStream_t stream, const int* tile_list,
                                  const int* tile_count) {
-  if (T == 64 || T == 128 || T == 256) {
-    if (tile_list || !fused_supported(T, C, G, P)) return hipErrorInvalidValue;
+  if (T == 64 || T == 128 || T == 256 || (T == 32 && P == kStaged32)) {
+    if (tile_list || !staged_shape(T, C, G, P)) return hipErrorInvalidValue;
     if (colsum) {
       const hipError_t e = launch_colsum(x, dtype, stride, T, N, colsum, stream);
       if (e != hipSuccess) return e;

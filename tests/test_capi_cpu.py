@@ -46,6 +46,21 @@ def test_null_arguments_are_rejected():
     assert L.sc_writer_record_hit(None, None, None) == sc.SC_ERR_IO
 
 
+@pytest.mark.parametrize("T", [2, 3, 4, 8, 13, 16, 32, 33, 64, 100, 128, 200, 256])
+def test_default_kernel_shape_is_compiled(T):
+    # sc_dd_create checks, before any device call, that the default one-pass
+    # shape for T_r <= 256 is a compiled kernel (otherwise the search would
+    # fall back to the global-memory staged path). Without a GPU the create
+    # then fails at its first allocation; with one it succeeds.
+    L = sc.lib()
+    h = ctypes.c_void_p()
+    status = L.sc_dd_create(T, 4096, -2.79e-6, 18.25, 0, ctypes.byref(h))
+    assert status in (sc.SC_OK, sc.SC_ERR_HIP, sc.SC_ERR_OUT_OF_MEMORY), (
+        status, L.sc_last_error_string())
+    if status == sc.SC_OK:
+        L.sc_dd_destroy(h)
+
+
 def _meta():
     return sc.FilterbankMetadata(fch1=8419.319795910269, foff=-2.7939677238464355e-06,
                                  coarse_channel_size=1 << 20, source_name="Voyager1",

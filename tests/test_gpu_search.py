@@ -211,19 +211,20 @@ def test_one_pass_t32_shapes_are_bit_identical(gpu, oracle, shape, N, max_drift)
     assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
 
 
-@pytest.mark.parametrize("T", [64, 128, 256])
+@pytest.mark.parametrize("T", [32, 64, 128, 256])
 @pytest.mark.parametrize("G", [1, 4, 32])
 @pytest.mark.parametrize("N,max_drift", [(5000, 2.0), (700, 3.0)])
 def test_staged_one_pass_is_bit_identical(gpu, oracle, T, G, N, max_drift):
     # The one-pass kernel for 64..256 time steps (level-16 trees staged in LDS
-    # by groups of offsets) with 1, 4 and 32 drift blocks per workgroup, on a
-    # channel range wider than a tile and on one narrower than the drift span.
+    # by groups of offsets; at 32 selected by prefetch 8) with 1, 4 and 32
+    # drift blocks per workgroup, on a channel range wider than a tile and on
+    # one narrower than the drift span.
     print("staged T=%d G=%d N=%d" % (T, G, N), flush=True)
     rng = np.random.default_rng(T + N)
     tones = [(rng.uniform(0, N), rng.uniform(-2, 2), rng.uniform(1, 3)) for _ in range(6)]
     data = _waterfall(T, N, 91 + T, tones)
     hits, st, ref = _compare_search(oracle, data, T, -2.79e-6, 18.25, max_drift, 0.0, 5.0,
-                                    tuning=(4, G, 0))
+                                    tuning=(4, G, 8 if T == 32 else 0))
     np.testing.assert_array_equal(st["column_sums"], ref.column_sums)
     np.testing.assert_array_equal(st["top_sums"], ref.top_sums)
     pos = ref.top_sums > 0
