@@ -417,3 +417,58 @@ def test_fil_load_and_condition_on_device(gpu, oracle, tmp_path):
     ref = oracle.search(host, T, r.foff, r.tsamp, 1.0, 0.0, 10.0, fch1=r.fch1, num_threads=8)
     assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
     assert all(h.index != N // 2 for h in hits)  # the DC spike is gone
+
+
+SNR_CONFIGS = [
+    # (name, T, N, max_drift, tone drift range Hz/s, tone SNR range, gain, taper, dc spike)
+    ("config 1: 16 x 2^20, 4 Hz/s", 16, 1 << 20, 4.0, (0.2, 4.0), (10, 100), 1.0, 0.0, 0.0),
+    ("config 2 coarse channel: 16 x 2^20, 10 Hz/s, gain, taper, DC", 16, 1 << 20, 10.0,
+     (0.5, 10.0), (8, 200), 1.7, 0.05, 50.0),
+    ("config 3 beam: 32 x 2^18, 5 Hz/s", 32, 1 << 18, 5.0, (0.2, 5.0), (10, 50), 1.0, 0.0, 0.0),
+    ("config 3 beam: 64 x 2^17, 5 Hz/s", 64, 1 << 17, 5.0, (0.2, 5.0), (10, 50), 1.0, 0.0, 0.0),
+    ("config 3 beam: 256 x 2^17, 5 Hz/s", 256, 1 << 17, 5.0, (0.2, 5.0), (10, 50), 1.0, 0.0, 0.0),
+]
+
+
+@pytest.mark.parametrize("cfg", SNR_CONFIGS, ids=[c[0].split(":")[0] + " T=%d" % c[1]
+                                                  for c in SNR_CONFIGS])
+def test_snr_error_against_fp64_per_config(gpu, oracle, cfg):
+    # Every hit's SNR against the FP64 oracle's (FP64 path sum of the hit's
+    # (block, offset), FP64 column-sum median and trimmed deviation), in both
+    # statistics modes. Host-exact mode inherits the reference's float
+    # accumulation of the deviation (~1e-5 relative); device mode forms the
+    # moments in FP64.
+    name, T, N, max_drift, (d0, d1), (s0, s1), gain, taper, dc = cfg
+    foff, tsamp = -2.79e-6, 18.25
+    torch = _torch()
+    rng = np.random.default_rng(T * 7 + N % 1000)
+    tones = []
+    for _ in range(20):
+        drift = rng.uniform(d0, d1) * rng.choice([-1, 1])
+        amp = rng.uniform(s0, s1) * gain / math.sqrt(T)
+        tones.append((float(rng.uniform(2000, N - 2000)), float(drift * tsamp / (foff * 1e6)),
+                      float(amp)))
+    x = torch.empty((T, N), dtype=torch.float32, device="cuda")
+    sc.gen_waterfall(x, T, seed=4242, gain=gain, taper_frac=taper, dc_spike=dc, tones=tones)
+    data = x.cpu().numpy()
+    med64, _, std64 = oracle.stats_fp64(data)
+    buf = sc.FilterbankBuffer(T, N, data=x)
+    meta = sc.FilterbankMetadata(fch1=8000.0, foff=foff, coarse_channel_size=N)
+    worst = {}
+    for mode in (sc.SC_STATS_HOST_EXACT, sc.SC_STATS_DEVICE):
+        dd = sc.Dedopplerer(T, N, foff, tsamp)
+        dd.set_stats_mode(mode)
+        hits = []
+        dd.search(buf, meta, -1, 0, max_drift, 0.0, 10.0, hits)
+        st = dd.state()
+        assert len(hits) >= 2
+        errs = []
+        for h in hits:
+            b, p = int(st["top_blocks"][h.index]), int(st["top_offsets"][h.index])
+            snr64 = (oracle.path_sum_fp64(data, h.index, b, p) - med64) / std64
+            errs.append(abs(h.snr - snr64) / abs(snr64))
+        worst[mode] = max(errs)
+    print("%s: max relative SNR error vs FP64: host-exact %.2e, device %.2e"
+          % (name, worst[sc.SC_STATS_HOST_EXACT], worst[sc.SC_STATS_DEVICE]), flush=True)
+    assert worst[sc.SC_STATS_DEVICE] <= 1e-5
+    assert worst[sc.SC_STATS_HOST_EXACT] <= 5e-4
