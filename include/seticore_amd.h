@@ -155,7 +155,10 @@ sc_status sc_dd_copy_hits(const sc_dd* dd, sc_hit* hits, size_t capacity,
  * num_channels elements each (any may be NULL): column sums (dedoppler.cu:207),
  * top path sums, their drift blocks and path offsets (dedoppler.cu:32-55),
  * and the median/std the threshold used. No counterpart (the reference keeps
- * them in private pinned arrays); used to check parity. */
+ * them in private pinned arrays); used to check parity. After a
+ * device-statistics search at rounded_timesteps <= 16 the path offset is
+ * materialised only for each window's candidate (the channels hits can come
+ * from); other channels with a positive sum read -1 there. */
 sc_status sc_dd_copy_state(const sc_dd* dd, float* column_sums, float* top_sums,
                            int* top_drift_blocks, int* top_path_offsets,
                            float* median, float* std_dev);

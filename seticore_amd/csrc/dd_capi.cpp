@@ -544,9 +544,14 @@ namespace {
 // handle's second stream while the path-sum kernels run on `stream` (they
 // need only the column sums); `stream` waits for them at the end, so the
 // tail is then just the window merge.
+//
+// Searches that end in the device tail carry block keys at T_r <= 16
+// (block_key_search): the tail recovers the candidates' offsets.
+bool block_key_search(const sc_dd* dd) { return dd->rounded <= 16; }
+
 sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_stride,
                             const DriftRange& r, hipStream_t stream,
-                            bool overlap_stats = false) {
+                            bool overlap_stats = false, bool device_tail = false) {
   const int T = dd->rounded, N = dd->num_channels;
   const int nblocks = r.bmax - r.bmin + 1;
   // dedoppler.cu:205 (the reference zeroes the float sums; the key 0 means "no
@@ -567,7 +572,8 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
   if (one_pass) {
     SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
                                      dd->group, dd->prefetch, dd->top_key,
-                                     overlap_stats ? nullptr : dd->colsum, stream),
+                                     overlap_stats ? nullptr : dd->colsum, stream, nullptr,
+                                     nullptr, device_tail && block_key_search(dd)),
            "fused tree");
   } else if (sc::two_pass_supported(T, dd->pass_group)) {
     if (!overlap_stats)
@@ -598,10 +604,12 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
   return SC_OK;
 }
 
-// Device statistics, window merge and the copies of the per-window candidates.
+// Device statistics, window merge (with the candidates' offsets after a
+// block-key search of `data`) and the copies of the per-window candidates.
 sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_threshold,
-                              hipStream_t stream, int4* h_cand = nullptr,
-                              float* h_stats = nullptr, bool stats_done = false) {
+                              const void* data, int dtype, long row_stride, hipStream_t stream,
+                              int4* h_cand = nullptr, float* h_stats = nullptr,
+                              bool stats_done = false) {
   if (!h_cand) h_cand = dd->h_cand;
   if (!h_stats) h_stats = dd->h_stats;
   const int T = dd->rounded, N = dd->num_channels;
@@ -609,11 +617,16 @@ sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_thresho
     SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
                                    dd->stats_out, stream),
            "device stats");
+  const bool bk = block_key_search(dd);
   SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
                                  dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off, dd->cand,
-                                 stream),
+                                 bk, stream),
          "window merge");
   const int nwin = (int)(((long)N + r.window - 1) / r.window);
+  if (bk)
+    SC_HIP(sc::launch_candidate_offsets(data, dtype, row_stride, T, N, nwin, dd->cand,
+                                        dd->top_off, stream),
+           "candidate offsets");
   SC_HIP(hipMemcpyAsync(h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost, stream),
          "copy candidates");
   SC_HIP(hipMemcpyAsync(h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
@@ -639,9 +652,11 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
     SC_HIP(hipStreamCreateWithFlags(&dd->capture_stream, hipStreamNonBlocking), "stream");
   SC_HIP(hipStreamBeginCapture(dd->capture_stream, hipStreamCaptureModeThreadLocal),
          "begin capture");
-  sc_status s = enqueue_path_sums(dd, data, dtype, row_stride, r, dd->capture_stream, true);
+  sc_status s =
+      enqueue_path_sums(dd, data, dtype, row_stride, r, dd->capture_stream, true, true);
   if (s == SC_OK)
-    s = enqueue_device_tail(dd, r, snr_threshold, dd->capture_stream, nullptr, nullptr, true);
+    s = enqueue_device_tail(dd, r, snr_threshold, data, dtype, row_stride, dd->capture_stream,
+                            nullptr, nullptr, true);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(dd->capture_stream, &graph);
   if (s != SC_OK) {
@@ -774,8 +789,10 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
       if (s != SC_OK) return s;
       SC_HIP(hipGraphLaunch(exec, stream), "graph launch");
     } else {
-      s = enqueue_path_sums(dd, data, dtype, row_stride, r, stream, true);
-      if (s == SC_OK) s = enqueue_device_tail(dd, r, snr_threshold, stream, nullptr, nullptr, true);
+      s = enqueue_path_sums(dd, data, dtype, row_stride, r, stream, true, true);
+      if (s == SC_OK)
+        s = enqueue_device_tail(dd, r, snr_threshold, data, dtype, row_stride, stream, nullptr,
+                                nullptr, true);
       if (s != SC_OK) return s;
     }
     SC_HIP(hipStreamSynchronize(stream), "search");
@@ -855,14 +872,17 @@ sc_status sc_dd_search_proposed(sc_dd* dd, sc_pn* pn, const void* data, int dtyp
          "tile list");
   SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
                                    dd->group, dd->prefetch, dd->top_key, nullptr, stream,
-                                   dd->gate_list, dd->gate_count),
+                                   dd->gate_list, dd->gate_count, true),
          "gated tree");
   SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
                                  dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off, dd->cand,
-                                 stream),
+                                 true, stream),
          "window merge");
-  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
   const int nwin = (int)(((long)N + r.window - 1) / r.window);
+  SC_HIP(sc::launch_candidate_offsets(data, dtype, row_stride, T, N, nwin, dd->cand, dd->top_off,
+                                      stream),
+         "candidate offsets");
+  if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
   SC_HIP(hipMemcpyAsync(dd->h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost,
                         stream), "copy candidates");
   SC_HIP(hipMemcpyAsync(dd->h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
@@ -1237,7 +1257,7 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
       if (last_tail[bi.shape]) e = hipStreamWaitEvent(b->capture_stream, last_tail[bi.shape], 0);
       if (e != hipSuccess) break;
       s = enqueue_path_sums(dd, bi.in.data, bi.in.dtype, bi.in.row_stride, bi.r,
-                            b->capture_stream);
+                            b->capture_stream, false, true);
       if (s != SC_OK) break;
       hipEvent_t sums = nullptr, tail = nullptr;
       e = new_event(&sums);
@@ -1245,8 +1265,9 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
       if (e == hipSuccess) e = hipEventRecord(sums, b->capture_stream);
       if (e == hipSuccess) e = hipStreamWaitEvent(b->tail_stream, sums, 0);
       if (e != hipSuccess) break;
-      s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, b->tail_stream,
-                              b->h_cand + bi.cand_offset, b->h_stats + 2 * i);
+      s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, bi.in.data, bi.in.dtype,
+                              bi.in.row_stride, b->tail_stream, b->h_cand + bi.cand_offset,
+                              b->h_stats + 2 * i);
       if (s != SC_OK) break;
       e = hipEventRecord(tail, b->tail_stream);
       last_tail[bi.shape] = tail;

@@ -21,12 +21,14 @@ typedef unsigned long long top_key_t;
 // prefetch the next window into registers. Writes column sums (unless colsum
 // is null) and max-merges top keys. With tile_list (device), only the channel
 // tiles tile_list[0 .. *tile_count) are searched (tiles of
-// fused_tile_width(T, C) channels).
+// fused_tile_width(T, C) channels). With block_keys (T_r <= 16 only; other T_r
+// ignore it) the keys are block keys: (sum bits << 32) | ~(drift block -
+// bmin), the offset left to launch_candidate_offsets after the window merge.
 hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, int N,
                                  int bmin, int nblocks, int C, int G, int P,
                                  top_key_t* top_key, float* colsum, hipStream_t stream,
                                  const int* tile_list = nullptr,
-                                 const int* tile_count = nullptr);
+                                 const int* tile_count = nullptr, bool block_keys = false);
 int fused_tile_width(int T, int C);
 
 // Whether launch_fused_tree_top has a kernel for (T, C, G, P). At T = 32 the
@@ -81,10 +83,16 @@ hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsign
 
 // Per-window merge (dedoppler.cu:273-303) fused with unpacking the keys into
 // the three top arrays. cand_out[i] = {index or -1, sum bits, drift block,
-// path offset} for window i.
+// path offset} for window i. With block keys the offsets (unpacked and in
+// the candidates) are -1 until launch_candidate_offsets fills the candidates'.
 hipError_t launch_window_merge(const top_key_t* key, int N, int T, int bmin, int window,
                                double snr_threshold, const float* stats, float* top_sum,
-                               int* top_blk, int* top_off, int4* cand_out, hipStream_t stream);
+                               int* top_blk, int* top_off, int4* cand_out, bool block_keys,
+                               hipStream_t stream);
+// After a block-key merge (T_r <= 16): each candidate's path offset, from the
+// input of the search.
+hipError_t launch_candidate_offsets(const void* x, int dtype, long stride, int T, int N,
+                                    int nwin, int4* cand, int* top_off, hipStream_t stream);
 
 // Gathers path sums for (offset, channel) pairs from a [T][N] buffer.
 hipError_t launch_gather(const float* sums, int N, const int* offsets, const int* chans,

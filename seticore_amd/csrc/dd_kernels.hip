@@ -76,6 +76,18 @@ __device__ __forceinline__ float key_sum(unsigned long long k) {
   return __uint_as_float((unsigned)(k >> 29));
 }
 
+// Block keys (the T_r <= 16 kernel's partial results): (sum bits, ~block) with
+// block = drift_block - min_drift_block, for sums > 0 only. Their unsigned
+// order is (sum, -block), so a 64-bit atomic max keeps the largest sum and,
+// among equal sums, the lowest block; fixup_offsets_kernel then finds the
+// offset and writes the full key.
+__device__ __forceinline__ unsigned long long block_key(float s, int block) {
+  return ((unsigned long long)__float_as_uint(s) << 32) | (unsigned)~(unsigned)block;
+}
+__device__ __forceinline__ float block_key_sum(unsigned long long k) {
+  return __uint_as_float((unsigned)(k >> 32));
+}
+
 // ---------------------------------------------------------------------------
 // Register-resident Taylor tree.
 //
@@ -242,13 +254,52 @@ __device__ __forceinline__ bool block_interior(int c_w, int b, int N) {
   return (c_w + (T - 1) * b >= 0) && (c_w + Sh::kOut - 1 + (T - 1) * b + (T - 1) < N);
 }
 
-// One drift block: tree, edge masking when the block is not interior for this
-// tile (dedoppler.cu:40-46, including the early return that skips every offset
-// of a block whose first path ends left of channel 0), then the key max in
-// ascending (block, offset) order.
-template <int T, int C, int G, int gp>
+// Running best of the T_r <= 16 kernel, per channel held by a lane. Full
+// keys (BK false): the key max in ascending (block, offset) order, one
+// v_max_f64 per path sum. Block keys (BK true): the largest sum and its block
+// only - a block replaces the best when its largest sum is strictly larger
+// (blocks come in ascending order), which is the reference's rule
+// (dedoppler.cu:40,49,214) up to the offset within the block; the offset of a
+// hit is recovered afterwards (candidate_offsets_kernel). About a third less
+// VALU work per path sum.
+template <int C, bool BK>
+struct Best;
+template <int C>
+struct Best<C, false> {
+  double key[C];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int j = 0; j < C; ++j) key[j] = __longlong_as_double((long long)kEmptyKey);
+  }
+  __device__ __forceinline__ unsigned long long merge_key(int j) const {
+    const unsigned long long k = (unsigned long long)__double_as_longlong(key[j]);
+    return k > kEmptyKey ? k : 0;
+  }
+};
+template <int C>
+struct Best<C, true> {
+  float sum[C];
+  int block[C];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      sum[j] = 0.0f;
+      block[j] = 0;
+    }
+  }
+  __device__ __forceinline__ unsigned long long merge_key(int j) const {
+    return sum[j] > 0.0f ? block_key(sum[j], block[j]) : 0;
+  }
+};
+
+// One drift block (block index `block` from the minimum): tree, edge masking
+// when the block is not interior for this tile (dedoppler.cu:40-46, including
+// the early return that skips every offset of a block whose first path ends
+// left of channel 0; skipped paths become 0 and never register, as only sums
+// > 0 do, dedoppler.cu:205), then the running best.
+template <int T, int C, int G, bool BK, int gp>
 __device__ __forceinline__ void fused_block(lds_float* const (&rp)[T], int g, int ng, int b0,
-                                            int c_w, int N, unsigned low0, double (&best)[C]) {
+                                            int c_w, int N, int block0, Best<C, BK>& best) {
   if constexpr (gp < C) {
     if (g + gp < ng) {
       const int b = b0 + g + gp;
@@ -256,7 +307,6 @@ __device__ __forceinline__ void fused_block(lds_float* const (&rp)[T], int g, in
       read_block<T, C, G, gp>(rp, v);
       tree_regs<T, C>(v);
       if (!block_interior<T, C>(c_w, b, N)) {
-        // Skipped paths become 0: their keys never exceed kEmptyKey.
         const int lane = threadIdx.x;
 #pragma unroll
         for (int j = 0; j < C; ++j) {
@@ -268,15 +318,29 @@ __device__ __forceinline__ void fused_block(lds_float* const (&rp)[T], int g, in
             if (p >= pmax) v[p][j] = 0.0f;
         }
       }
-      const unsigned lowg = low0 - (unsigned)((g + gp) * T);
+      const int block = block0 + g + gp;
+      if constexpr (BK) {
 #pragma unroll
-      for (int p = 0; p < T; ++p) {
-        const unsigned low = lowg - (unsigned)p;
+        for (int j = 0; j < C; ++j) {
+          float m = v[0][j];
 #pragma unroll
-        for (int j = 0; j < C; ++j) best[j] = fmax(best[j], make_key(v[p][j], low));
+          for (int p = 1; p < T; ++p) m = fmaxf(m, v[p][j]);
+          if (m > best.sum[j]) {
+            best.sum[j] = m;
+            best.block[j] = block;
+          }
+        }
+      } else {
+        const unsigned lowg = (unsigned)kCodeMask - (unsigned)(block * T);
+#pragma unroll
+        for (int p = 0; p < T; ++p) {
+          const unsigned low = lowg - (unsigned)p;
+#pragma unroll
+          for (int j = 0; j < C; ++j) best.key[j] = fmax(best.key[j], make_key(v[p][j], low));
+        }
       }
     }
-    fused_block<T, C, G, gp + 1>(rp, g, ng, b0, c_w, N, low0, best);
+    fused_block<T, C, G, BK, gp + 1>(rp, g, ng, b0, c_w, N, block0, best);
   }
 }
 
@@ -300,7 +364,7 @@ __device__ __forceinline__ float ord_val(unsigned k) {
 // (workgroups go round-robin to the 8 XCDs) sweeps a contiguous range of
 // tiles. The workgroup whose chunk holds drift block 0 also writes the tile's
 // column sums in the reference's sequential order (dedoppler.cu:68-71).
-template <int T, int C, int G, int MODE, typename In>
+template <int T, int C, int G, int MODE, typename In, bool BK>
 __global__ void __launch_bounds__(kWave)
     fused_tree_top_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
                           int ngroups, int chunks, int groups_per_chunk, int total_items,
@@ -345,9 +409,8 @@ __global__ void __launch_bounds__(kWave)
   store_window<T, C, G>(win, pf);
   __syncthreads();
 
-  double best[C];
-#pragma unroll
-  for (int j = 0; j < C; ++j) best[j] = __longlong_as_double((long long)kEmptyKey);
+  Best<C, BK> best;
+  best.init();
 
   for (int grp = g_begin; grp < g_end; ++grp) {
     // MODE 0: every work item is one drift group (the launcher makes
@@ -357,11 +420,10 @@ __global__ void __launch_bounds__(kWave)
       fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + (grp + 1) * G, 0);
     const int b0 = bmin + grp * G;
     const int ng = min(G, nblocks - grp * G);
-    const unsigned low0 = (unsigned)kCodeMask - (unsigned)(grp * G * T);
     lds_float* rp[T];
     row_bases<T, C, G>(win, rp);
     for (int g = 0; g < ng; g += C) {
-      fused_block<T, C, G, 0>(rp, g, ng, b0, c_w, N, low0, best);
+      fused_block<T, C, G, BK, 0>(rp, g, ng, b0, c_w, N, grp * G, best);
 #pragma unroll
       for (int t = 1; t < T; ++t) rp[t] += t;  // next h step: rows shift by t columns
     }
@@ -377,9 +439,59 @@ __global__ void __launch_bounds__(kWave)
 #pragma unroll
     for (int j = 0; j < C; ++j) {
       const int c = c_w + lane * C + j;
-      const unsigned long long k = (unsigned long long)__double_as_longlong(best[j]);
-      if (c < N && k > kEmptyKey) atomicMax(&top_key[c], k);
+      const unsigned long long k = best.merge_key(j);
+      if (c < N && k != 0) atomicMax(&top_key[c], k);
     }
+  }
+}
+
+// Level-L entry (tb, p) of drift block b's tree at start channel c, added in
+// the register tree's order: entry (2tb, p/2) at c plus entry (2tb+1, p/2) at
+// c + (p+1)/2 (tree_stage); level 1 is row tb at c + tb*b (the shear), 0
+// outside [0, N) as in the staged windows.
+template <int L, typename In>
+__device__ __forceinline__ float tree_entry(const In* __restrict__ x, long stride, int N, int b,
+                                            int tb, int p, int c) {
+  if constexpr (L == 1) {
+    const long col = (long)c + (long)tb * b;
+    return (col >= 0 && col < N) ? to_f32<In>(x[(long)tb * stride + col]) : 0.0f;
+  } else {
+    const float lo = tree_entry<L / 2, In>(x, stride, N, b, 2 * tb, p >> 1, c);
+    const float hi = tree_entry<L / 2, In>(x, stride, N, b, 2 * tb + 1, p >> 1, c + ((p + 1) >> 1));
+    return lo + hi;
+  }
+}
+
+// Offset of each window's candidate after a block-key search: the candidate
+// record holds its channel, sum and drift block; the offset is the first
+// (lowest) one whose path sum in that block equals the sum (the reference
+// keeps the first of equal sums, dedoppler.cu:49). Also completes the
+// unpacked offset of that channel. T lanes per window, one offset each.
+template <int T, typename In>
+__global__ void __launch_bounds__(kWave)
+    candidate_offsets_kernel(const In* __restrict__ x, long stride, int N, int nwin,
+                             int4* __restrict__ cand, int* __restrict__ top_off) {
+  constexpr int kPerWave = kWave / T;
+  const int slot = threadIdx.x / T;
+  const int p = threadIdx.x % T;
+  const int w = blockIdx.x * kPerWave + slot;
+  int4 r = make_int4(-1, 0, 0, 0);
+  bool match = false;
+  if (w < nwin) {
+    r = cand[w];
+    if (r.x >= 0) {
+      const int last0 = r.x + (T - 1) * r.z;
+      const int pmax = last0 < 0 ? 0 : min(T, N - last0);
+      match = p < pmax && tree_entry<T, In>(x, stride, N, r.z, 0, p, r.x) == __int_as_float(r.y);
+    }
+  }
+  const unsigned long long all = __ballot(match);
+  const unsigned long long mine = (all >> (slot * T)) & ((T == 64) ? ~0ull : ((1ull << T) - 1));
+  if (p == 0 && w < nwin && r.x >= 0) {
+    const int p_best = mine ? __ffsll((long long)mine) - 1 : 0;
+    r.w = p_best;
+    cand[w] = r;
+    top_off[r.x] = p_best;
   }
 }
 
@@ -1419,7 +1531,7 @@ __global__ void __launch_bounds__(256)
     window_merge_kernel(const top_key_t* __restrict__ key, int N, int T, int bmin, int window,
                         double snr_threshold, const float* __restrict__ stats,
                         float* __restrict__ top_sum, int* __restrict__ top_blk,
-                        int* __restrict__ top_off, int4* __restrict__ out) {
+                        int* __restrict__ top_off, int4* __restrict__ out, int block_keys) {
   __shared__ float sv[4];
   __shared__ int si[4];
   __shared__ int any_larger;
@@ -1436,8 +1548,11 @@ __global__ void __launch_bounds__(256)
   int bi = -1;
   for (long f = lo + tid; f < hi; f += blockDim.x) {
     const top_key_t k = key[f];
-    const float v = k ? key_sum(k) : 0.0f;
-    if (k) {
+    const float v = k ? (block_keys ? block_key_sum(k) : key_sum(k)) : 0.0f;
+    if (k && block_keys) {
+      top_blk[f] = (int)~(unsigned)k + bmin;
+      top_off[f] = -1;  // completed for candidates by candidate_offsets_kernel
+    } else if (k) {
       const unsigned code = (unsigned)(kCodeMask - (k & kCodeMask));
       top_blk[f] = (int)(code / (unsigned)T) + bmin;
       top_off[f] = (int)(code % (unsigned)T);
@@ -1482,7 +1597,7 @@ __global__ void __launch_bounds__(256)
   int larger = 0;
   for (long f = s0 + tid; f < s1; f += blockDim.x) {
     const top_key_t k = key[f];
-    larger |= (k ? key_sum(k) : 0.0f) > bv;
+    larger |= (k ? (block_keys ? block_key_sum(k) : key_sum(k)) : 0.0f) > bv;
   }
   if (larger) any_larger = 1;
   __syncthreads();
@@ -1649,7 +1764,7 @@ inline int blocks_for(long n, int threads, int cap) {
 template <int T, int C, int G, int MODE, typename In>
 hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nblocks,
                               top_key_t* top_key, float* colsum, const int* tile_list,
-                              const int* tile_count, hipStream_t stream) {
+                              const int* tile_count, bool block_keys, hipStream_t stream) {
   using Sh = TileShape<T, C>;
   const long ntiles = ((long)N + Sh::kOut - 1) / Sh::kOut;
   const long ngroups = ((long)nblocks + G - 1) / G;
@@ -1666,9 +1781,15 @@ hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nbl
   const long total = ntiles * chunks;
   if (total > (1L << 30)) return hipErrorInvalidValue;
   const int grid = (int)((total + 7) / 8 * 8);
-  hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, MODE, In>), dim3(grid), dim3(kWave), 0,
-                     stream, x, stride, N, bmin, nblocks, (int)ngroups, (int)chunks,
-                     (int)per_chunk, (int)total, top_key, colsum, tile_list, tile_count);
+  if (block_keys)
+    hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, MODE, In, true>), dim3(grid), dim3(kWave),
+                       0, stream, x, stride, N, bmin, nblocks, (int)ngroups, (int)chunks,
+                       (int)per_chunk, (int)total, top_key, colsum, tile_list, tile_count);
+  else
+    hipLaunchKernelGGL((fused_tree_top_kernel<T, C, G, MODE, In, false>), dim3(grid),
+                       dim3(kWave), 0, stream, x, stride, N, bmin, nblocks, (int)ngroups,
+                       (int)chunks, (int)per_chunk, (int)total, top_key, colsum, tile_list,
+                       tile_count);
   return hipGetLastError();
 }
 
@@ -1692,11 +1813,12 @@ hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nbl
 template <typename In>
 hipError_t fused_dispatch(const In* x, long stride, int T, int N, int bmin, int nblocks, int C,
                           int G, int P, top_key_t* top_key, float* colsum, const int* tile_list,
-                          const int* tile_count, hipStream_t stream) {
+                          const int* tile_count, bool block_keys, hipStream_t stream) {
 #define SC_FUSED_CASE(TT, CC, GG, PP)                                                     \
   if (T == TT && C == CC && G == GG && P == PP)                                           \
     return launch_fused_typed<TT, CC, GG, PP, In>(x, stride, N, bmin, nblocks, top_key,      \
-                                                       colsum, tile_list, tile_count, stream);
+                                                       colsum, tile_list, tile_count,         \
+                                                       block_keys, stream);
   SC_FUSED_VARIANTS(SC_FUSED_CASE)
 #undef SC_FUSED_CASE
   return hipErrorInvalidValue;
@@ -1864,7 +1986,7 @@ bool fused_supported(int T, int C, int G, int P) {
 hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, int N, int bmin,
                                  int nblocks, int C, int G, int P, top_key_t* top_key,
                                  float* colsum, hipStream_t stream, const int* tile_list,
-                                 const int* tile_count) {
+                                 const int* tile_count, bool block_keys) {
   if (T == 64 || T == 128 || T == 256 || (T == 32 && P == kStaged32)) {
     if (tile_list || !staged_shape(T, C, G, P)) return hipErrorInvalidValue;
     if (colsum) {
@@ -1902,14 +2024,52 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
   switch (dtype) {
     case 0:
       return fused_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, C, G, P,
-                                   top_key, colsum, tile_list, tile_count, stream);
+                                   top_key, colsum, tile_list, tile_count, block_keys, stream);
     case 1:
       return fused_dispatch<__half>((const __half*)x, stride, T, N, bmin, nblocks, C, G, P,
-                                    top_key, colsum, tile_list, tile_count, stream);
+                                    top_key, colsum, tile_list, tile_count, block_keys, stream);
     case 2:
       return fused_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N, bmin,
                                             nblocks, C, G, P, top_key, colsum, tile_list,
-                                            tile_count, stream);
+                                            tile_count, block_keys, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int T, typename In>
+hipError_t candidate_offsets_typed(const In* x, long stride, int N, int nwin, int4* cand,
+                                   int* top_off, hipStream_t stream) {
+  constexpr int per_wave = kWave / T;
+  hipLaunchKernelGGL((candidate_offsets_kernel<T, In>), dim3((nwin + per_wave - 1) / per_wave),
+                     dim3(kWave), 0, stream, x, stride, N, nwin, cand, top_off);
+  return hipGetLastError();
+}
+
+template <typename In>
+hipError_t candidate_offsets_dispatch(const In* x, long stride, int T, int N, int nwin,
+                                      int4* cand, int* top_off, hipStream_t stream) {
+  switch (T) {
+    case 2: return candidate_offsets_typed<2, In>(x, stride, N, nwin, cand, top_off, stream);
+    case 4: return candidate_offsets_typed<4, In>(x, stride, N, nwin, cand, top_off, stream);
+    case 8: return candidate_offsets_typed<8, In>(x, stride, N, nwin, cand, top_off, stream);
+    case 16: return candidate_offsets_typed<16, In>(x, stride, N, nwin, cand, top_off, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_candidate_offsets(const void* x, int dtype, long stride, int T, int N,
+                                    int nwin, int4* cand, int* top_off, hipStream_t stream) {
+  if (nwin <= 0) return hipSuccess;
+  switch (dtype) {
+    case 0:
+      return candidate_offsets_dispatch<float>((const float*)x, stride, T, N, nwin, cand,
+                                               top_off, stream);
+    case 1:
+      return candidate_offsets_dispatch<__half>((const __half*)x, stride, T, N, nwin, cand,
+                                                top_off, stream);
+    case 2:
+      return candidate_offsets_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N,
+                                                        nwin, cand, top_off, stream);
   }
   return hipErrorInvalidValue;
 }
@@ -2030,10 +2190,12 @@ hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsign
 
 hipError_t launch_window_merge(const top_key_t* key, int N, int T, int bmin, int window,
                                double snr_threshold, const float* stats, float* top_sum,
-                               int* top_blk, int* top_off, int4* cand_out, hipStream_t stream) {
+                               int* top_blk, int* top_off, int4* cand_out, bool block_keys,
+                               hipStream_t stream) {
   const int nwin = (int)(((long)N + window - 1) / window);
   hipLaunchKernelGGL(window_merge_kernel, dim3(nwin), dim3(256), 0, stream, key, N, T, bmin,
-                     window, snr_threshold, stats, top_sum, top_blk, top_off, cand_out);
+                     window, snr_threshold, stats, top_sum, top_blk, top_off, cand_out,
+                     (int)block_keys);
   return hipGetLastError();
 }
 
