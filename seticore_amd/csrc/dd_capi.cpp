@@ -545,9 +545,13 @@ namespace {
 // need only the column sums); `stream` waits for them at the end, so the
 // tail is then just the window merge.
 //
-// Searches that end in the device tail carry block keys at T_r <= 16
-// (block_key_search): the tail recovers the candidates' offsets.
-bool block_key_search(const sc_dd* dd) { return dd->rounded <= 16; }
+// Searches that end in the device tail carry block keys when the one-pass
+// kernel is the T_r <= 16 or the staged one (block_key_search): the tail
+// recovers the candidates' offsets.
+bool block_key_search(const sc_dd* dd) {
+  return dd->rounded <= 16 ||
+         (!dd->use_two_pass && sc::staged_shape(dd->rounded, dd->lanes, dd->group, dd->prefetch));
+}
 
 sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_stride,
                             const DriftRange& r, hipStream_t stream,

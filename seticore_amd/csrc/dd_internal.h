@@ -21,8 +21,8 @@ typedef unsigned long long top_key_t;
 // prefetch the next window into registers. Writes column sums (unless colsum
 // is null) and max-merges top keys. With tile_list (device), only the channel
 // tiles tile_list[0 .. *tile_count) are searched (tiles of
-// fused_tile_width(T, C) channels). With block_keys (T_r <= 16 only; other T_r
-// ignore it) the keys are block keys: (sum bits << 32) | ~(drift block -
+// fused_tile_width(T, C) channels). With block_keys (T_r <= 16 and the staged
+// kernel; fused32_kernel ignores it) the keys are block keys: (sum bits << 32) | ~(drift block -
 // bmin), the offset left to launch_candidate_offsets after the window merge.
 hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, int N,
                                  int bmin, int nblocks, int C, int G, int P,
@@ -36,6 +36,8 @@ int fused_tile_width(int T, int C);
 // (shape (4, G, kStaged32)) instead of fused32_kernel.
 constexpr int kStaged32 = 8;
 bool fused_supported(int T, int C, int G, int P);
+// Whether (T, C, G, P) selects the LDS-staged kernel (T = 32..256).
+bool staged_shape(int T, int C, int G, int P);
 
 // T_r = 32..256: path sums of drift blocks [b_first, b_first + nb) folded into
 // the top keys by two passes through scratch (nb * T * row_pitch floats,

@@ -462,36 +462,77 @@ __device__ __forceinline__ float tree_entry(const In* __restrict__ x, long strid
   }
 }
 
+// Path (b, p) at start channel c of a T-row tree (any T up to 256), in the
+// tree's order: T/16 level-16 entries (tree_entry), each at the channel its
+// position in the upper levels puts it (the second half of a level-L entry
+// with offset q starts (q+1)/2 channels on), then summed pairwise level by
+// level. The level-16 entries are formed one at a time to keep few loads in
+// flight per lane.
+template <int T, typename In>
+__device__ float path_sum_exact(const In* __restrict__ x, long stride, int N, int b, int p,
+                                int c) {
+  if constexpr (T <= 16) {
+    return tree_entry<T, In>(x, stride, N, b, 0, p, c);
+  } else {
+    constexpr int K = T / 16;
+    constexpr int kLogK = K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : 4;
+    static_assert((1 << kLogK) == K, "T is a power of two up to 256");
+    float r[K];
+#pragma unroll 1
+    for (int n = 0; n < K; ++n) {
+      int cn = c, pl = p;
+      for (int lvl = kLogK - 1; lvl >= 0; --lvl) {
+        if ((n >> lvl) & 1) cn += (pl + 1) >> 1;
+        pl >>= 1;
+      }
+      // Rows 16n.. in the whole tree's shear: channel cn + 16n*b at row 16n.
+      r[n] = tree_entry<16, In>(x + (long)16 * n * stride, stride, N, b, 0, pl, cn + 16 * n * b);
+    }
+#pragma unroll
+    for (int width = K; width > 1; width >>= 1)
+#pragma unroll
+      for (int i = 0; i < width / 2; ++i) r[i] = r[2 * i] + r[2 * i + 1];
+    return r[0];
+  }
+}
+
 // Offset of each window's candidate after a block-key search: the candidate
 // record holds its channel, sum and drift block; the offset is the first
 // (lowest) one whose path sum in that block equals the sum (the reference
 // keeps the first of equal sums, dedoppler.cu:49). Also completes the
-// unpacked offset of that channel. T lanes per window, one offset each.
+// unpacked offset of that channel. min(T, 64) lanes per window, one offset
+// each per round.
 template <int T, typename In>
 __global__ void __launch_bounds__(kWave)
     candidate_offsets_kernel(const In* __restrict__ x, long stride, int N, int nwin,
                              int4* __restrict__ cand, int* __restrict__ top_off) {
-  constexpr int kPerWave = kWave / T;
-  const int slot = threadIdx.x / T;
-  const int p = threadIdx.x % T;
+  constexpr int kLanes = T < kWave ? T : kWave;
+  constexpr int kPerWave = kWave / kLanes;
+  const int slot = threadIdx.x / kLanes;
+  const int lane = threadIdx.x % kLanes;
   const int w = blockIdx.x * kPerWave + slot;
+  const unsigned long long group_mask = kLanes == kWave ? ~0ull : ((1ull << kLanes) - 1);
   int4 r = make_int4(-1, 0, 0, 0);
-  bool match = false;
+  int pmax = 0;
   if (w < nwin) {
     r = cand[w];
     if (r.x >= 0) {
       const int last0 = r.x + (T - 1) * r.z;
-      const int pmax = last0 < 0 ? 0 : min(T, N - last0);
-      match = p < pmax && tree_entry<T, In>(x, stride, N, r.z, 0, p, r.x) == __int_as_float(r.y);
+      pmax = last0 < 0 ? 0 : min(T, N - last0);
     }
   }
-  const unsigned long long all = __ballot(match);
-  const unsigned long long mine = (all >> (slot * T)) & ((T == 64) ? ~0ull : ((1ull << T) - 1));
-  if (p == 0 && w < nwin && r.x >= 0) {
-    const int p_best = mine ? __ffsll((long long)mine) - 1 : 0;
-    r.w = p_best;
+  int p_best = -1;
+  for (int p0 = 0; p0 < T; p0 += kLanes) {  // uniform trip count: every lane takes the ballot
+    const int p = p0 + lane;
+    const bool match =
+        p < pmax && path_sum_exact<T, In>(x, stride, N, r.z, p, r.x) == __int_as_float(r.y);
+    const unsigned long long mine = (__ballot(match) >> (slot * kLanes)) & group_mask;
+    if (p_best < 0 && mine) p_best = p0 + __ffsll((long long)mine) - 1;
+  }
+  if (lane == 0 && w < nwin && r.x >= 0) {
+    r.w = p_best < 0 ? 0 : p_best;
     cand[w] = r;
-    top_off[r.x] = p_best;
+    top_off[r.x] = r.w;
   }
 }
 
@@ -1040,10 +1081,10 @@ __device__ __forceinline__ void staged_phase1(float (&v)[16][Staged<M>::C], int 
 
 // Phase 2 for one offset q = q0 + qq: R_q[0..M-1] from LDS, the M-row tree at
 // drift 0, masking (dedoppler.cu:40-46) and the keys.
-template <int M>
+template <int M, bool BK>
 __device__ __forceinline__ void staged_phase2(const lds_float* R, int qq, int q, int b, int c_w,
-                                              int N, bool interior, unsigned low_b,
-                                              double (&best)[Staged<M>::C]) {
+                                              int N, bool interior, int block,
+                                              Best<Staged<M>::C, BK>& best) {
   using St = Staged<M>;
   constexpr int C = St::C;
   constexpr int T = St::T;
@@ -1066,19 +1107,32 @@ __device__ __forceinline__ void staged_phase2(const lds_float* R, int qq, int q,
         if (q * M + r >= pmax) v[r][j] = 0.0f;
     }
   }
-  const unsigned low0 = low_b - (unsigned)(q * M);
+  if constexpr (BK) {
 #pragma unroll
-  for (int r = 0; r < M; ++r) {
-    const unsigned low = low0 - (unsigned)r;
+    for (int j = 0; j < C; ++j) {
+      float m = v[0][j];
 #pragma unroll
-    for (int j = 0; j < C; ++j) best[j] = fmax(best[j], make_key(v[r][j], low));
+      for (int r = 1; r < M; ++r) m = fmaxf(m, v[r][j]);
+      if (m > best.sum[j]) {
+        best.sum[j] = m;
+        best.block[j] = block;
+      }
+    }
+  } else {
+    const unsigned low0 = (unsigned)kCodeMask - (unsigned)(block * T + q * M);
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      const unsigned low = low0 - (unsigned)r;
+#pragma unroll
+      for (int j = 0; j < C; ++j) best.key[j] = fmax(best.key[j], make_key(v[r][j], low));
+    }
   }
 }
 
-template <int M, int G_IDX, typename In>
+template <int M, int G_IDX, typename In, bool BK>
 __device__ __forceinline__ void staged_groups(const In* __restrict__ x, long stride, int N, int b,
-                                              int c_w, bool interior, unsigned low_b,
-                                              lds_float* R, double (&best)[Staged<M>::C]) {
+                                              int c_w, bool interior, int block, lds_float* R,
+                                              Best<Staged<M>::C, BK>& best) {
   using St = Staged<M>;
   if constexpr (G_IDX < St::kGroups) {
     constexpr int Q0 = G_IDX * St::QG;
@@ -1092,17 +1146,18 @@ __device__ __forceinline__ void staged_groups(const In* __restrict__ x, long str
     }
     __syncthreads();
     for (int qq = wave; qq < St::QG; qq += St::kWaves)
-      staged_phase2<M>(R, qq, Q0 + qq, b, c_w, N, interior, low_b, best);
+      staged_phase2<M, BK>(R, qq, Q0 + qq, b, c_w, N, interior, block, best);
     __syncthreads();
-    staged_groups<M, G_IDX + 1, In>(x, stride, N, b, c_w, interior, low_b, R, best);
+    staged_groups<M, G_IDX + 1, In, BK>(x, stride, N, b, c_w, interior, block, R, best);
   }
 }
 
 // Grid: one workgroup of 4 waves per (channel tile, chunk of G consecutive
 // drift blocks; G at run time), tile-major and dealt so that each XCD takes a contiguous
-// range of items. The waves keep their own key maxima over the chunk and merge
-// them through LDS before one atomic max per channel.
-template <int M, typename In>
+// range of items. The waves keep their own maxima over the chunk (full keys,
+// or block keys as in fused_tree_top_kernel) and merge them through LDS
+// before one atomic max per channel.
+template <int M, typename In, bool BK>
 __global__ void __launch_bounds__(kWave * Staged<M>::kWaves, Staged<M>::kMinWaves)
     staged_tree_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
                        int G, int nchunks, int total_items, top_key_t* __restrict__ top_key) {
@@ -1119,34 +1174,34 @@ __global__ void __launch_bounds__(kWave * Staged<M>::kWaves, Staged<M>::kMinWave
   const int c_w = tile * St::kOut;
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
-  double best[C];
-#pragma unroll
-  for (int j = 0; j < C; ++j) best[j] = __longlong_as_double((long long)kEmptyKey);
+  Best<C, BK> best;
+  best.init();
   const bool tile_in = c_w + St::kOut - 1 < N;
   const int g_end = min(nblocks, (chunk + 1) * G);
   for (int g = chunk * G; g < g_end; ++g) {
     const int b = bmin + g;
     const bool interior =
         tile_in && (c_w + (T - 1) * b >= 0) && (c_w + St::kOut - 1 + (T - 1) * b + (T - 1) < N);
-    const unsigned low_b = (unsigned)kCodeMask - (unsigned)(g * T);
-    staged_groups<M, 0, In>(x, stride, N, b, c_w, interior, low_b, R, best);
+    staged_groups<M, 0, In, BK>(x, stride, N, b, c_w, interior, g, R, best);
   }
-  // The last phase 2 ended with a barrier: R is free for the merge.
+  // The last phase 2 ended with a barrier: R is free for the merge. Keys of
+  // either kind read as non-negative FP64 numbers in their unsigned order.
   double* red = (double*)r_storage;
   if (wave > 0) {
 #pragma unroll
-    for (int j = 0; j < C; ++j) red[(wave - 1) * kWave * C + lane * C + j] = best[j];
+    for (int j = 0; j < C; ++j)
+      red[(wave - 1) * kWave * C + lane * C + j] = __longlong_as_double((long long)best.merge_key(j));
   }
   __syncthreads();
   if (wave == 0 && lane * C < St::kOut) {
 #pragma unroll
     for (int j = 0; j < C; ++j) {
+      double acc = __longlong_as_double((long long)best.merge_key(j));
 #pragma unroll
-      for (int w = 0; w < St::kWaves - 1; ++w)
-        best[j] = fmax(best[j], red[w * kWave * C + lane * C + j]);
+      for (int w = 0; w < St::kWaves - 1; ++w) acc = fmax(acc, red[w * kWave * C + lane * C + j]);
       const int c = c_w + lane * C + j;
-      const unsigned long long key = (unsigned long long)__double_as_longlong(best[j]);
-      if (c < N && key > kEmptyKey) atomicMax(&top_key[c], key);
+      const unsigned long long key = (unsigned long long)__double_as_longlong(acc);
+      if (c < N && key != 0) atomicMax(&top_key[c], key);
     }
   }
 }
@@ -1911,27 +1966,33 @@ hipError_t two_pass_dispatch(const In* x, long stride, int T, int N, int bmin, i
 
 template <int M, typename In>
 hipError_t launch_staged_typed(const In* x, long stride, int N, int bmin, int nblocks, int G,
-                               top_key_t* top_key, hipStream_t stream) {
+                               top_key_t* top_key, bool block_keys, hipStream_t stream) {
   using St = Staged<M>;
   const long ntiles = ((long)N + St::kOut - 1) / St::kOut;
   const long nchunks = ((long)nblocks + G - 1) / G;
   const long total = ntiles * nchunks;
   if (total > (1L << 30)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((staged_tree_kernel<M, In>), dim3((int)((total + 7) / 8 * 8)),
-                     dim3(kWave * St::kWaves), 0, stream, x, stride, N, bmin, nblocks, G,
-                     (int)nchunks, (int)total, top_key);
+  if (block_keys)
+    hipLaunchKernelGGL((staged_tree_kernel<M, In, true>), dim3((int)((total + 7) / 8 * 8)),
+                       dim3(kWave * St::kWaves), 0, stream, x, stride, N, bmin, nblocks, G,
+                       (int)nchunks, (int)total, top_key);
+  else
+    hipLaunchKernelGGL((staged_tree_kernel<M, In, false>), dim3((int)((total + 7) / 8 * 8)),
+                       dim3(kWave * St::kWaves), 0, stream, x, stride, N, bmin, nblocks, G,
+                       (int)nchunks, (int)total, top_key);
   return hipGetLastError();
 }
 
 template <typename In>
 hipError_t staged_dispatch(const In* x, long stride, int T, int N, int bmin, int nblocks, int G,
-                           top_key_t* top_key, hipStream_t stream) {
+                           top_key_t* top_key, bool bk, hipStream_t stream) {
   switch (T) {
-    case 32: return launch_staged_typed<2, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
-    case 64: return launch_staged_typed<4, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
-    case 128: return launch_staged_typed<8, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
+    case 32: return launch_staged_typed<2, In>(x, stride, N, bmin, nblocks, G, top_key, bk, stream);
+    case 64: return launch_staged_typed<4, In>(x, stride, N, bmin, nblocks, G, top_key, bk, stream);
+    case 128:
+      return launch_staged_typed<8, In>(x, stride, N, bmin, nblocks, G, top_key, bk, stream);
     case 256:
-      return launch_staged_typed<16, In>(x, stride, N, bmin, nblocks, G, top_key, stream);
+      return launch_staged_typed<16, In>(x, stride, N, bmin, nblocks, G, top_key, bk, stream);
   }
   return hipErrorInvalidValue;
 }
@@ -1996,13 +2057,13 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
     switch (dtype) {
       case 0:
         return staged_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, G, top_key,
-                                      stream);
+                                      block_keys, stream);
       case 1:
         return staged_dispatch<__half>((const __half*)x, stride, T, N, bmin, nblocks, G,
-                                       top_key, stream);
+                                       top_key, block_keys, stream);
       case 2:
         return staged_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N, bmin,
-                                               nblocks, G, top_key, stream);
+                                               nblocks, G, top_key, block_keys, stream);
     }
     return hipErrorInvalidValue;
   }
@@ -2039,7 +2100,7 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
 template <int T, typename In>
 hipError_t candidate_offsets_typed(const In* x, long stride, int N, int nwin, int4* cand,
                                    int* top_off, hipStream_t stream) {
-  constexpr int per_wave = kWave / T;
+  constexpr int per_wave = kWave / (T < kWave ? T : kWave);
   hipLaunchKernelGGL((candidate_offsets_kernel<T, In>), dim3((nwin + per_wave - 1) / per_wave),
                      dim3(kWave), 0, stream, x, stride, N, nwin, cand, top_off);
   return hipGetLastError();
@@ -2053,6 +2114,12 @@ hipError_t candidate_offsets_dispatch(const In* x, long stride, int T, int N, in
     case 4: return candidate_offsets_typed<4, In>(x, stride, N, nwin, cand, top_off, stream);
     case 8: return candidate_offsets_typed<8, In>(x, stride, N, nwin, cand, top_off, stream);
     case 16: return candidate_offsets_typed<16, In>(x, stride, N, nwin, cand, top_off, stream);
+    case 32: return candidate_offsets_typed<32, In>(x, stride, N, nwin, cand, top_off, stream);
+    case 64: return candidate_offsets_typed<64, In>(x, stride, N, nwin, cand, top_off, stream);
+    case 128:
+      return candidate_offsets_typed<128, In>(x, stride, N, nwin, cand, top_off, stream);
+    case 256:
+      return candidate_offsets_typed<256, In>(x, stride, N, nwin, cand, top_off, stream);
   }
   return hipErrorInvalidValue;
 }

@@ -472,3 +472,39 @@ def test_snr_error_against_fp64_per_config(gpu, oracle, cfg):
           % (name, worst[sc.SC_STATS_HOST_EXACT], worst[sc.SC_STATS_DEVICE]), flush=True)
     assert worst[sc.SC_STATS_DEVICE] <= 1e-5
     assert worst[sc.SC_STATS_HOST_EXACT] <= 5e-4
+
+
+@pytest.mark.parametrize("T", [4, 16, 32, 64, 128, 256])
+def test_block_key_offsets_match_full_keys(gpu, oracle, T):
+    # Device-statistics searches at these T_r carry block keys (the largest sum
+    # per drift block) and recover each candidate's path offset afterwards;
+    # host-exact searches carry full (sum, block, offset) keys. Every hit both
+    # report must agree on channel, drift, power and offset, and the device
+    # search's offsets at its hits must be the oracle's.
+    N = 20000 if T <= 64 else 8000
+    print("block keys T=%d N=%d" % (T, N), flush=True)
+    rng = np.random.default_rng(T + 3)
+    tones = [(rng.uniform(0, N), rng.uniform(-1.5, 1.5), rng.uniform(2, 4)) for _ in range(12)]
+    data = _waterfall(T, N, 60 + T, tones)
+    foff, tsamp, max_drift = -2.79e-6, 18.25, 1.0 if T <= 64 else 0.5
+    buf = sc.FilterbankBuffer(T, N, data=_to_gpu(data))
+    meta = sc.FilterbankMetadata(fch1=8000.0, foff=foff, coarse_channel_size=N)
+    got = {}
+    states = {}
+    for mode in (sc.SC_STATS_HOST_EXACT, sc.SC_STATS_DEVICE):
+        dd = sc.Dedopplerer(T, N, foff, tsamp)
+        dd.set_stats_mode(mode)
+        hits = []
+        dd.search(buf, meta, -1, 0, max_drift, 0.0, 5.0, hits)
+        got[mode] = {h.index: (h.drift_steps, h.power) for h in hits}
+        states[mode] = dd.state()
+    exact, dev = got[sc.SC_STATS_HOST_EXACT], got[sc.SC_STATS_DEVICE]
+    common = set(exact) & set(dev)
+    assert len(common) >= 2
+    for i in common:
+        assert exact[i] == dev[i], i
+    ref = oracle.search(data, T, foff, tsamp, max_drift, 0.0, 5.0, num_threads=8)
+    st = states[sc.SC_STATS_DEVICE]
+    for i in dev:
+        assert st["top_offsets"][i] == ref.top_offsets[i], i
+        assert st["top_blocks"][i] == ref.top_blocks[i], i
