@@ -1128,10 +1128,10 @@ extern "C" sc_status sc_condition_coarse_channel(float* data, int num_timesteps,
 // instance's statistics/merge/copy tail hangs off its path sums on a second
 // chain, so it can overlap the next instance's path sums. Instances of one
 // shape (rounded time steps, channels, foff, tsamp) share search handles:
-// two, used alternately, for shapes up to 16 time steps (where the tail is a
-// noticeable share of a search: measured +10 % at 16 x 2^20; at 32 steps the
-// overlap slowed the LDS-bound one-pass kernel), one otherwise; an instance's
-// path sums wait for the tail of the previous instance on the same handle.
+// two, used alternately (measured +10 % at 16 x 2^20 and +1.6 % on the uneven
+// 16..256 batch), or one for a shape tuned to the two-pass search (whose
+// scratch is large); an instance's path sums wait for the tail of the
+// previous instance on the same handle.
 // Each instance's candidates land in its own pinned host area.
 
 struct sc_batch {
@@ -1200,7 +1200,11 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
     }
     if (shape < 0) {
       shape = (int)b->shapes.size();
-      const int copies = T <= 16 ? 2 : 1;
+      int copies = 2;
+      for (int k = 0; k < num_tuning; ++k)
+        if (tuning[k].rounded_timesteps == T && tuning[k].num_channels == in.num_channels &&
+            T > 16 && tuning[k].pass_group != 0)
+          copies = 1;
       for (int c = 0; c < copies && s == SC_OK; ++c) {
         sc_dd* dd = nullptr;
         s = sc_dd_create(in.num_timesteps, in.num_channels, in.foff, in.tsamp, 0, &dd);
