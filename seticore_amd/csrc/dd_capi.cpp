@@ -56,8 +56,6 @@ int round_up_pow2(int n) {
   return r;
 }
 
-size_t dtype_size(int dtype) { return dtype == SC_F32 ? 4 : 2; }
-
 // Default fused-kernel shape per T_r: channels per lane and drift blocks per
 // LDS window.
 void default_tuning(int T, int* lanes, int* group, int* prefetch) {
@@ -187,37 +185,37 @@ sc_status host_alloc(void** p, size_t bytes, const char* what) {
 }
 
 void release(sc_dd* dd) {
-  hipFree(dd->top_key);
-  hipFree(dd->colsum);
-  hipFree(dd->top_sum);
-  hipFree(dd->top_blk);
-  hipFree(dd->top_off);
-  hipFree(dd->stats_scratch);
-  hipFree(dd->stats_hist);
-  hipFree(dd->stats_out);
-  hipFree(dd->cand);
-  hipFree(dd->tree_a);
-  hipFree(dd->tree_b);
-  hipFree(dd->pass_scratch);
-  hipFree(dd->gate_list);
-  hipFree(dd->gate_count);
-  hipHostFree(dd->h_gate);
-  hipFree(dd->gather_idx);
-  hipFree(dd->gather_out);
-  hipHostFree(dd->h_colsum);
-  hipHostFree(dd->h_top_sum);
-  hipHostFree(dd->h_top_blk);
-  hipHostFree(dd->h_top_off);
-  hipHostFree(dd->h_cand);
-  hipHostFree(dd->h_stats);
-  if (dd->ev_start) hipEventDestroy(dd->ev_start);
-  if (dd->ev_stop) hipEventDestroy(dd->ev_stop);
-  for (auto& g : dd->graph_cache) hipGraphExecDestroy(g.exec);
+  (void)hipFree(dd->top_key);
+  (void)hipFree(dd->colsum);
+  (void)hipFree(dd->top_sum);
+  (void)hipFree(dd->top_blk);
+  (void)hipFree(dd->top_off);
+  (void)hipFree(dd->stats_scratch);
+  (void)hipFree(dd->stats_hist);
+  (void)hipFree(dd->stats_out);
+  (void)hipFree(dd->cand);
+  (void)hipFree(dd->tree_a);
+  (void)hipFree(dd->tree_b);
+  (void)hipFree(dd->pass_scratch);
+  (void)hipFree(dd->gate_list);
+  (void)hipFree(dd->gate_count);
+  (void)hipHostFree(dd->h_gate);
+  (void)hipFree(dd->gather_idx);
+  (void)hipFree(dd->gather_out);
+  (void)hipHostFree(dd->h_colsum);
+  (void)hipHostFree(dd->h_top_sum);
+  (void)hipHostFree(dd->h_top_blk);
+  (void)hipHostFree(dd->h_top_off);
+  (void)hipHostFree(dd->h_cand);
+  (void)hipHostFree(dd->h_stats);
+  if (dd->ev_start) (void)hipEventDestroy(dd->ev_start);
+  if (dd->ev_stop) (void)hipEventDestroy(dd->ev_stop);
+  for (auto& g : dd->graph_cache) (void)hipGraphExecDestroy(g.exec);
   dd->graph_cache.clear();
-  if (dd->capture_stream) hipStreamDestroy(dd->capture_stream);
-  if (dd->aux_stream) hipStreamDestroy(dd->aux_stream);
-  if (dd->ev_fork) hipEventDestroy(dd->ev_fork);
-  if (dd->ev_join) hipEventDestroy(dd->ev_join);
+  if (dd->capture_stream) (void)hipStreamDestroy(dd->capture_stream);
+  if (dd->aux_stream) (void)hipStreamDestroy(dd->aux_stream);
+  if (dd->ev_fork) (void)hipEventDestroy(dd->ev_fork);
+  if (dd->ev_join) (void)hipEventDestroy(dd->ev_join);
 }
 
 struct DriftRange {
@@ -257,7 +255,7 @@ int pass_blocks(const sc_dd* dd) {
 sc_status ensure_pass_scratch(sc_dd* dd, int nb) {
   if (dd->pass_scratch && dd->pass_alloc_blocks >= nb) return SC_OK;
   if (dd->pass_scratch) {
-    hipFree(dd->pass_scratch);
+    (void)hipFree(dd->pass_scratch);
     dd->device_bytes -= sizeof(float) * (size_t)dd->pass_alloc_blocks * dd->rounded *
                         (size_t)dd->pass_pitch;
     dd->pass_scratch = nullptr;
@@ -423,7 +421,7 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
     delete dd;
     return fail(SC_ERR_UNSUPPORTED, "no compiled default kernel shape for this T_r");
   }
-  hipGetDevice(&dd->device);
+  (void)hipGetDevice(&dd->device);
 
   const size_t N = num_channels;
   sc_status s = SC_OK;
@@ -525,7 +523,7 @@ sc_status sc_dd_set_two_pass(sc_dd* dd, int drift_group, int blocks_per_launch) 
   const int nb = pass_blocks(dd);
   if (nb > dd->pass_alloc_blocks) {
     // Cached graphs hold the old scratch address.
-    for (auto& g : dd->graph_cache) hipGraphExecDestroy(g.exec);
+    for (auto& g : dd->graph_cache) (void)hipGraphExecDestroy(g.exec);
     dd->graph_cache.clear();
     return ensure_pass_scratch(dd, nb);
   }
@@ -664,16 +662,16 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(dd->capture_stream, &graph);
   if (s != SC_OK) {
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return s;
   }
   if (e != hipSuccess) return hip_fail(e, "end capture");
   hipGraphExec_t exec = nullptr;
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
+  (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return hip_fail(e, "instantiate graph");
   if (dd->graph_cache.size() >= sc_dd::kMaxGraphs) {
-    hipGraphExecDestroy(dd->graph_cache.front().exec);
+    (void)hipGraphExecDestroy(dd->graph_cache.front().exec);
     dd->graph_cache.erase(dd->graph_cache.begin());
   }
   dd->graph_cache.push_back({data, dtype, row_stride, r.bmin, nblocks, r.window, snr_threshold,
@@ -992,8 +990,8 @@ sc_status sc_dd_add_incoherent_power(sc_dd* dd, const void* data, int dtype,
     return a.drift_steps < b.drift_steps;
   });
   if (num_hits > dd->gather_capacity) {
-    hipFree(dd->gather_idx);
-    hipFree(dd->gather_out);
+    (void)hipFree(dd->gather_idx);
+    (void)hipFree(dd->gather_out);
     dd->gather_idx = nullptr;
     dd->gather_out = nullptr;
     s = dev_alloc(dd, (void**)&dd->gather_idx, 2 * num_hits * sizeof(int), "gather");
@@ -1051,7 +1049,7 @@ sc_status sc_normalize(const float* data, int rounded_timesteps, int num_channel
          "normalize scratch");
   hipError_t e = hipMalloc((void**)&moments, 2 * sizeof(double));
   if (e != hipSuccess) {
-    hipFree(scratch);
+    (void)hipFree(scratch);
     return hip_fail(e, "normalize moments");
   }
   e = sc::launch_normalize(data, rounded_timesteps, num_channels, row_stride, out_dtype, clamp,
@@ -1060,8 +1058,8 @@ sc_status sc_normalize(const float* data, int rounded_timesteps, int num_channel
   if (e == hipSuccess)
     e = hipMemcpyAsync(h, moments, sizeof(h), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  hipFree(scratch);
-  hipFree(moments);
+  (void)hipFree(scratch);
+  (void)hipFree(moments);
   if (e != hipSuccess) return hip_fail(e, "normalize");
   if (mean_out) *mean_out = h[0];
   if (std_out) *std_out = h[1];
@@ -1087,7 +1085,7 @@ sc_status sc_gen_waterfall(float* out, int num_timesteps, int rounded_timesteps,
     std::memcpy(h.data() + 2 * num_tones, tone_amp, sizeof(float) * num_tones);
     hipError_t e = hipMemcpy(dev, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-      hipFree(dev);
+      (void)hipFree(dev);
       return hip_fail(e, "tone params");
     }
   }
@@ -1096,7 +1094,7 @@ sc_status sc_gen_waterfall(float* out, int num_timesteps, int rounded_timesteps,
                                           dev ? dev + num_tones : nullptr,
                                           dev ? dev + 2 * num_tones : nullptr, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  hipFree(dev);
+  (void)hipFree(dev);
   if (e != hipSuccess) return hip_fail(e, "generate waterfall");
   return SC_OK;
 }
@@ -1156,12 +1154,12 @@ struct sc_batch {
 namespace {
 
 void batch_release(sc_batch* b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->capture_stream) hipStreamDestroy(b->capture_stream);
-  if (b->tail_stream) hipStreamDestroy(b->tail_stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  if (b->tail_stream) (void)hipStreamDestroy(b->tail_stream);
   for (sc_dd* dd : b->shapes) sc_dd_destroy(dd);
-  hipHostFree(b->h_cand);
-  hipHostFree(b->h_stats);
+  (void)hipHostFree(b->h_cand);
+  (void)hipHostFree(b->h_stats);
 }
 
 }  // namespace
@@ -1292,14 +1290,14 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
     }
     if (s == SC_OK) {
       size_t n = 0;
-      hipGraphGetNodes(graph, nullptr, &n);
+      (void)hipGraphGetNodes(graph, nullptr, &n);
       b->nodes = (int)n;
       e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
       if (e != hipSuccess) s = hip_fail(e, "instantiate graph");
     }
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
   }
-  for (hipEvent_t ev : events) hipEventDestroy(ev);
+  for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
   if (s != SC_OK) {
     batch_release(b);
     delete b;

@@ -257,7 +257,7 @@ sc_status sc_fil_load_coarse_channel(const sc_fil* h, int i, int rounded_timeste
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (e != hipSuccess) s = fail(SC_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
   }
-  hipHostFree(staging);
+  (void)hipHostFree(staging);
   return s;
 }
 

@@ -80,8 +80,8 @@ struct Cleanup {
     for (File& f : files) {
       if (f.fil) sc_fil_close(f.fil);
       if (f.dd) sc_dd_destroy(f.dd);
-      if (f.device) hipFree(f.device);
-      if (f.host) hipHostFree(f.host);
+      if (f.device) (void)hipFree(f.device);
+      if (f.host) (void)hipHostFree(f.host);
     }
     if (writer) sc_writer_close(writer);
   }

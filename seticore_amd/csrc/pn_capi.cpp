@@ -44,22 +44,22 @@ sc_status alloc(sc_pn* pn, void** p, size_t bytes, const char* what) {
 }
 
 void release(sc_pn* pn) {
-  hipFree(pn->w.params);
-  hipFree(pn->w.w1b);
-  hipFree(pn->w.w2b);
-  hipFree(pn->w.w3b);
-  hipFree(pn->a.h1);
-  hipFree(pn->a.h2);
-  hipFree(pn->a.h3);
-  hipFree(pn->adam_m);
-  hipFree(pn->adam_v);
-  hipFree(pn->z);
-  hipFree(pn->logits);
-  hipFree(pn->argmax);
-  hipFree(pn->dz);
-  hipFree(pn->gpart);
-  hipFree(pn->grad);
-  hipFree(pn->loss);
+  (void)hipFree(pn->w.params);
+  (void)hipFree(pn->w.w1b);
+  (void)hipFree(pn->w.w2b);
+  (void)hipFree(pn->w.w3b);
+  (void)hipFree(pn->a.h1);
+  (void)hipFree(pn->a.h2);
+  (void)hipFree(pn->a.h3);
+  (void)hipFree(pn->adam_m);
+  (void)hipFree(pn->adam_v);
+  (void)hipFree(pn->z);
+  (void)hipFree(pn->logits);
+  (void)hipFree(pn->argmax);
+  (void)hipFree(pn->dz);
+  (void)hipFree(pn->gpart);
+  (void)hipFree(pn->grad);
+  (void)hipFree(pn->loss);
 }
 
 // splitmix64
@@ -91,7 +91,7 @@ sc_status sc_pn_create(int max_batch, sc_pn** out) {
     return fail(SC_ERR_INVALID_ARGUMENT, "max_batch must be in [1, 65536]");
   sc_pn* pn = new sc_pn();
   pn->max_batch = max_batch;
-  hipGetDevice(&pn->device);
+  (void)hipGetDevice(&pn->device);
   const size_t B = max_batch;
   sc_status s = SC_OK;
   if (s == SC_OK) s = alloc(pn, (void**)&pn->w.params, kNumParams * 4, "parameters");
