@@ -65,6 +65,26 @@ def test_batch_equals_separate_searches(gpu):
         torch.cuda.synchronize()
 
 
+def test_two_pass_and_fused32_tunings_in_a_batch(gpu):
+    # Shapes tuned to the kernels that are not the defaults - the two-pass
+    # search at 64..256 time steps (full keys, one search handle per shape) and
+    # the register-parked one-pass kernel at 32 - must give the default
+    # (staged, block-key) searches' hits.
+    import torch
+    data, inst = _instances(torch)
+    want = _reference_hits(data, inst)
+    tuning = []
+    for T_r, N in sorted({tuple(x.shape) for x in data}):
+        if T_r == 32:
+            tuning.append(B.ScTuning(T_r, N, 2, 4, 2, 0, 0))
+        elif T_r >= 64:
+            tuning.append(B.ScTuning(T_r, N, 4, 8, 0, 4, 2))
+    b = B.Batch(inst, tuning)
+    for _ in range(2):
+        b.run()
+        assert _batch_hits(b) == want
+
+
 def test_autotuned_batch_equals_separate_searches(gpu):
     import torch
     data, inst = _instances(torch)
