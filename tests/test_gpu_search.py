@@ -474,18 +474,29 @@ def test_snr_error_against_fp64_per_config(gpu, oracle, cfg):
     assert worst[sc.SC_STATS_HOST_EXACT] <= 5e-4
 
 
+@pytest.mark.parametrize("ties", [False, True])
 @pytest.mark.parametrize("T", [4, 16, 32, 64, 128, 256])
-def test_block_key_offsets_match_full_keys(gpu, oracle, T):
+def test_block_key_offsets_match_full_keys(gpu, oracle, T, ties):
     # Device-statistics searches at these T_r carry block keys (the largest sum
     # per drift block) and recover each candidate's path offset afterwards;
     # host-exact searches carry full (sum, block, offset) keys. Every hit both
     # report must agree on channel, drift, power and offset, and the device
-    # search's offsets at its hits must be the oracle's.
+    # search's offsets at its hits must be the oracle's. With small-integer
+    # samples equal path sums are common, within and across drift blocks, so
+    # the lowest-(block, offset) rule is exercised.
     N = 20000 if T <= 64 else 8000
-    print("block keys T=%d N=%d" % (T, N), flush=True)
+    print("block keys T=%d N=%d ties=%s" % (T, N, ties), flush=True)
     rng = np.random.default_rng(T + 3)
     tones = [(rng.uniform(0, N), rng.uniform(-1.5, 1.5), rng.uniform(2, 4)) for _ in range(12)]
-    data = _waterfall(T, N, 60 + T, tones)
+    if ties:
+        data = rng.integers(0, 3, size=(T, N)).astype(np.float32)
+        for start, slope, _ in tones:
+            for t in range(T):
+                c = int(math.floor(start + slope * t + 0.5))
+                if 0 <= c < N:
+                    data[t, c] += 3.0
+    else:
+        data = _waterfall(T, N, 60 + T, tones)
     foff, tsamp, max_drift = -2.79e-6, 18.25, 1.0 if T <= 64 else 0.5
     buf = sc.FilterbankBuffer(T, N, data=_to_gpu(data))
     meta = sc.FilterbankMetadata(fch1=8000.0, foff=foff, coarse_channel_size=N)
