@@ -76,11 +76,12 @@ __device__ __forceinline__ float key_sum(unsigned long long k) {
   return __uint_as_float((unsigned)(k >> 29));
 }
 
-// Block keys (the T_r <= 16 kernel's partial results): (sum bits, ~block) with
-// block = drift_block - min_drift_block, for sums > 0 only. Their unsigned
-// order is (sum, -block), so a 64-bit atomic max keeps the largest sum and,
-// among equal sums, the lowest block; fixup_offsets_kernel then finds the
-// offset and writes the full key.
+// Block keys (device-statistics searches with the one-pass kernels):
+// (sum bits, ~block) with block = drift_block - min_drift_block, for sums > 0
+// only. Their unsigned order is (sum, -block), so a 64-bit atomic max keeps
+// the largest sum and, among equal sums, the lowest block; the window merge
+// reads the sums, and candidate_offsets_kernel finds the offset of each
+// window's candidate.
 __device__ __forceinline__ unsigned long long block_key(float s, int block) {
   return ((unsigned long long)__float_as_uint(s) << 32) | (unsigned)~(unsigned)block;
 }
