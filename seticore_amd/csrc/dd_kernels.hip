@@ -1825,10 +1825,12 @@ hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nbl
   const long ntiles = ((long)N + Sh::kOut - 1) / Sh::kOut;
   const long ngroups = ((long)nblocks + G - 1) / G;
   // MODE 1/2: chunks of drift groups per tile, as long as the chip is still
-  // filled several times over (256 CUs x a few resident workgroups).
+  // filled several times over (256 CUs x a few resident workgroups). Measured
+  // at 16 x 2^20, 132 blocks, mode 2: 4096 / 8192 / 16384 / 32768 / 65536
+  // items -> 0.64 / 0.56 / 0.54 / 0.49 / 0.49 ms.
   long chunks = ngroups;
   if (MODE != 0) {
-    const long target = MODE == 1 ? 8192 : 16384;
+    const long target = MODE == 1 ? 8192 : 32768;
     chunks = 1;
     while (ntiles * chunks < target && chunks < ngroups) ++chunks;
   }
