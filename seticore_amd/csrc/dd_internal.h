@@ -36,7 +36,8 @@ int fused_tile_width(int T, int C);
 // (shape (4, G, kStaged32)) instead of fused32_kernel.
 constexpr int kStaged32 = 8;
 bool fused_supported(int T, int C, int G, int P);
-// Whether (T, C, G, P) selects the LDS-staged kernel (T = 32..256).
+// Whether (T, C, G, P) selects the LDS-staged kernel (T = 32..256). At
+// T = 128, 256, P = 1 gives each offset group its own workgroup.
 bool staged_shape(int T, int C, int G, int P);
 
 // T_r = 32..256: path sums of drift blocks [b_first, b_first + nb) folded into

@@ -1130,35 +1130,62 @@ __device__ __forceinline__ void staged_phase2(const lds_float* R, int qq, int q,
   }
 }
 
+// One offset group of one drift block: phase 1, barrier, phase 2, barrier.
+template <int M, int G_IDX, typename In, bool BK>
+__device__ __forceinline__ void staged_group(const In* __restrict__ x, long stride, int N, int b,
+                                             int c_w, bool interior, int block, lds_float* R,
+                                             Best<Staged<M>::C, BK>& best) {
+  using St = Staged<M>;
+  constexpr int Q0 = G_IDX * St::QG;
+  const int wave = threadIdx.x / kWave;
+  // (Double-buffering the rows of the next time block measured slower: the
+  // extra registers halve the waves per SIMD.)
+  for (int k = wave; k < M; k += St::kWaves) {
+    float v[16][St::C];
+    staged_load<M, In>(x, stride, N, k, b, (long)c_w + (long)k * (16 * b + Q0), v);
+    staged_phase1<M, G_IDX>(v, k, R);
+  }
+  __syncthreads();
+  for (int qq = wave; qq < St::QG; qq += St::kWaves)
+    staged_phase2<M, BK>(R, qq, Q0 + qq, b, c_w, N, interior, block, best);
+  __syncthreads();
+}
+
+// Every offset group of one drift block, in order.
 template <int M, int G_IDX, typename In, bool BK>
 __device__ __forceinline__ void staged_groups(const In* __restrict__ x, long stride, int N, int b,
                                               int c_w, bool interior, int block, lds_float* R,
                                               Best<Staged<M>::C, BK>& best) {
-  using St = Staged<M>;
-  if constexpr (G_IDX < St::kGroups) {
-    constexpr int Q0 = G_IDX * St::QG;
-    const int wave = threadIdx.x / kWave;
-    // (Double-buffering the rows of the next time block measured slower: the
-    // extra registers halve the waves per SIMD.)
-    for (int k = wave; k < M; k += St::kWaves) {
-      float v[16][St::C];
-      staged_load<M, In>(x, stride, N, k, b, (long)c_w + (long)k * (16 * b + Q0), v);
-      staged_phase1<M, G_IDX>(v, k, R);
-    }
-    __syncthreads();
-    for (int qq = wave; qq < St::QG; qq += St::kWaves)
-      staged_phase2<M, BK>(R, qq, Q0 + qq, b, c_w, N, interior, block, best);
-    __syncthreads();
+  if constexpr (G_IDX < Staged<M>::kGroups) {
+    staged_group<M, G_IDX, In, BK>(x, stride, N, b, c_w, interior, block, R, best);
     staged_groups<M, G_IDX + 1, In, BK>(x, stride, N, b, c_w, interior, block, R, best);
   }
+}
+
+// Offset group grp (uniform over the workgroup) of one drift block.
+template <int M, int G_IDX, typename In, bool BK>
+__device__ __forceinline__ void staged_group_at(int grp, const In* __restrict__ x, long stride,
+                                                int N, int b, int c_w, bool interior, int block,
+                                                lds_float* R, Best<Staged<M>::C, BK>& best) {
+  if constexpr (G_IDX + 1 < Staged<M>::kGroups) {
+    if (grp != G_IDX) {
+      staged_group_at<M, G_IDX + 1, In, BK>(grp, x, stride, N, b, c_w, interior, block, R, best);
+      return;
+    }
+  }
+  staged_group<M, G_IDX, In, BK>(x, stride, N, b, c_w, interior, block, R, best);
 }
 
 // Grid: one workgroup of 4 waves per (channel tile, chunk of G consecutive
 // drift blocks; G at run time), tile-major and dealt so that each XCD takes a contiguous
 // range of items. The waves keep their own maxima over the chunk (full keys,
 // or block keys as in fused_tree_top_kernel) and merge them through LDS
-// before one atomic max per channel.
-template <int M, typename In, bool BK>
+// before one atomic max per channel. With SPLIT, an item is one offset group
+// of a (tile, chunk): the groups of one (tile, chunk) are consecutive items
+// of one XCD, dispatched together, so the rows each group reads (mostly the
+// same lines, shifted by k * q0 channels) are shared through that XCD's L2
+// instead of being read once per group after other workgroups evicted them.
+template <int M, typename In, bool BK, bool SPLIT>
 __global__ void __launch_bounds__(kWave * Staged<M>::kWaves, Staged<M>::kMinWaves)
     staged_tree_kernel(const In* __restrict__ x, long stride, int N, int bmin, int nblocks,
                        int G, int nchunks, int total_items, top_key_t* __restrict__ top_key) {
@@ -1170,8 +1197,10 @@ __global__ void __launch_bounds__(kWave * Staged<M>::kWaves, Staged<M>::kMinWave
   const int per_xcd = gridDim.x / 8;
   const int item = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (item >= total_items) return;
-  const int tile = item / nchunks;
-  const int chunk = item % nchunks;
+  const int grp = SPLIT ? item % St::kGroups : 0;
+  const int tc = SPLIT ? item / St::kGroups : item;
+  const int tile = tc / nchunks;
+  const int chunk = tc % nchunks;
   const int c_w = tile * St::kOut;
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
@@ -1183,7 +1212,10 @@ __global__ void __launch_bounds__(kWave * Staged<M>::kWaves, Staged<M>::kMinWave
     const int b = bmin + g;
     const bool interior =
         tile_in && (c_w + (T - 1) * b >= 0) && (c_w + St::kOut - 1 + (T - 1) * b + (T - 1) < N);
-    staged_groups<M, 0, In, BK>(x, stride, N, b, c_w, interior, g, R, best);
+    if constexpr (SPLIT)
+      staged_group_at<M, 0, In, BK>(grp, x, stride, N, b, c_w, interior, g, R, best);
+    else
+      staged_groups<M, 0, In, BK>(x, stride, N, b, c_w, interior, g, R, best);
   }
   // The last phase 2 ended with a barrier: R is free for the merge. Keys of
   // either kind read as non-negative FP64 numbers in their unsigned order.
@@ -1967,35 +1999,46 @@ hipError_t two_pass_dispatch(const In* x, long stride, int T, int N, int bmin, i
   return hipErrorInvalidValue;
 }
 
-template <int M, typename In>
+template <int M, typename In, bool SPLIT>
 hipError_t launch_staged_typed(const In* x, long stride, int N, int bmin, int nblocks, int G,
                                top_key_t* top_key, bool block_keys, hipStream_t stream) {
   using St = Staged<M>;
   const long ntiles = ((long)N + St::kOut - 1) / St::kOut;
   const long nchunks = ((long)nblocks + G - 1) / G;
-  const long total = ntiles * nchunks;
+  const long total = ntiles * nchunks * (SPLIT ? St::kGroups : 1);
   if (total > (1L << 30)) return hipErrorInvalidValue;
   if (block_keys)
-    hipLaunchKernelGGL((staged_tree_kernel<M, In, true>), dim3((int)((total + 7) / 8 * 8)),
-                       dim3(kWave * St::kWaves), 0, stream, x, stride, N, bmin, nblocks, G,
-                       (int)nchunks, (int)total, top_key);
+    hipLaunchKernelGGL((staged_tree_kernel<M, In, true, SPLIT>),
+                       dim3((int)((total + 7) / 8 * 8)), dim3(kWave * St::kWaves), 0, stream, x,
+                       stride, N, bmin, nblocks, G, (int)nchunks, (int)total, top_key);
   else
-    hipLaunchKernelGGL((staged_tree_kernel<M, In, false>), dim3((int)((total + 7) / 8 * 8)),
-                       dim3(kWave * St::kWaves), 0, stream, x, stride, N, bmin, nblocks, G,
-                       (int)nchunks, (int)total, top_key);
+    hipLaunchKernelGGL((staged_tree_kernel<M, In, false, SPLIT>),
+                       dim3((int)((total + 7) / 8 * 8)), dim3(kWave * St::kWaves), 0, stream, x,
+                       stride, N, bmin, nblocks, G, (int)nchunks, (int)total, top_key);
   return hipGetLastError();
 }
 
+// split: one workgroup per offset group (T_r = 128, 256: shape (4, G, 1)).
 template <typename In>
 hipError_t staged_dispatch(const In* x, long stride, int T, int N, int bmin, int nblocks, int G,
-                           top_key_t* top_key, bool bk, hipStream_t stream) {
+                           bool split, top_key_t* top_key, bool bk, hipStream_t stream) {
   switch (T) {
-    case 32: return launch_staged_typed<2, In>(x, stride, N, bmin, nblocks, G, top_key, bk, stream);
-    case 64: return launch_staged_typed<4, In>(x, stride, N, bmin, nblocks, G, top_key, bk, stream);
+    case 32:
+      return launch_staged_typed<2, In, false>(x, stride, N, bmin, nblocks, G, top_key, bk,
+                                               stream);
+    case 64:
+      return launch_staged_typed<4, In, false>(x, stride, N, bmin, nblocks, G, top_key, bk,
+                                               stream);
     case 128:
-      return launch_staged_typed<8, In>(x, stride, N, bmin, nblocks, G, top_key, bk, stream);
+      return split ? launch_staged_typed<8, In, true>(x, stride, N, bmin, nblocks, G, top_key,
+                                                      bk, stream)
+                   : launch_staged_typed<8, In, false>(x, stride, N, bmin, nblocks, G, top_key,
+                                                       bk, stream);
     case 256:
-      return launch_staged_typed<16, In>(x, stride, N, bmin, nblocks, G, top_key, bk, stream);
+      return split ? launch_staged_typed<16, In, true>(x, stride, N, bmin, nblocks, G, top_key,
+                                                       bk, stream)
+                   : launch_staged_typed<16, In, false>(x, stride, N, bmin, nblocks, G, top_key,
+                                                        bk, stream);
   }
   return hipErrorInvalidValue;
 }
@@ -2029,7 +2072,8 @@ hipError_t tree_blocks_dispatch(const In* x, long stride, int T, int N, int drif
 
 bool staged_shape(int T, int C, int G, int P) {
   const bool g_ok = G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32;
-  if (T == 64 || T == 128 || T == 256) return C == 4 && P == 0 && g_ok;
+  if (T == 64) return C == 4 && P == 0 && g_ok;
+  if (T == 128 || T == 256) return C == 4 && (P == 0 || P == 1) && g_ok;
   return T == 32 && C == 4 && P == kStaged32 && g_ok;
 }
 
@@ -2059,14 +2103,14 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
     }
     switch (dtype) {
       case 0:
-        return staged_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, G, top_key,
-                                      block_keys, stream);
+        return staged_dispatch<float>((const float*)x, stride, T, N, bmin, nblocks, G, P == 1,
+                                      top_key, block_keys, stream);
       case 1:
         return staged_dispatch<__half>((const __half*)x, stride, T, N, bmin, nblocks, G,
-                                       top_key, block_keys, stream);
+                                       P == 1, top_key, block_keys, stream);
       case 2:
         return staged_dispatch<__hip_bfloat16>((const __hip_bfloat16*)x, stride, T, N, bmin,
-                                               nblocks, G, top_key, block_keys, stream);
+                                               nblocks, G, P == 1, top_key, block_keys, stream);
     }
     return hipErrorInvalidValue;
   }

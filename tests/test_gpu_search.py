@@ -233,6 +233,37 @@ def test_staged_one_pass_is_bit_identical(gpu, oracle, T, G, N, max_drift):
     assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
 
 
+@pytest.mark.parametrize("T", [128, 256])
+@pytest.mark.parametrize("G", [1, 4, 32])
+@pytest.mark.parametrize("mode", [sc.SC_STATS_HOST_EXACT, sc.SC_STATS_DEVICE])
+def test_staged_split_groups_is_bit_identical(gpu, oracle, T, G, mode):
+    # Shape (4, G, 1) at 128 and 256 time steps: one workgroup per offset
+    # group, the groups' key maxima merged by atomics. Full keys (host-exact)
+    # and block keys with candidate offsets (device statistics).
+    N, max_drift = 5000, 2.0
+    print("staged split T=%d G=%d mode=%d" % (T, G, mode), flush=True)
+    rng = np.random.default_rng(T + G)
+    tones = [(rng.uniform(0, N), rng.uniform(-2, 2), rng.uniform(1, 3)) for _ in range(6)]
+    data = _waterfall(T, N, 17 + T, tones)
+    hits, st, ref = _compare_search(oracle, data, T, -2.79e-6, 18.25, max_drift, 0.0, 5.0,
+                                    stats_mode=mode, tuning=(4, G, 1))
+    np.testing.assert_array_equal(st["column_sums"], ref.column_sums)
+    np.testing.assert_array_equal(st["top_sums"], ref.top_sums)
+    pos = ref.top_sums > 0
+    np.testing.assert_array_equal(st["top_blocks"][pos], ref.top_blocks[pos])
+    if mode == sc.SC_STATS_HOST_EXACT:
+        np.testing.assert_array_equal(st["top_offsets"][pos], ref.top_offsets[pos])
+        assert [_hit_fields(h) for h in hits] == [_hit_fields(h) for h in ref.hits]
+    else:
+        # Offsets are recovered at the candidates only; hits equal the
+        # oracle's up to the device statistics' rounding of the threshold.
+        cand = st["top_offsets"] >= 0
+        np.testing.assert_array_equal(st["top_offsets"][cand], ref.top_offsets[cand])
+        want = {(h["index"], h["drift_steps"]) for h in ref.hits}
+        got = {(h.index, h.drift_steps) for h in hits}
+        assert got and len(got ^ want) <= 1
+
+
 def test_device_stats_mode(gpu, oracle):
     T, N = 16, 1 << 18
     rng = np.random.default_rng(5)
