@@ -926,8 +926,8 @@ __global__ void __launch_bounds__(kWave * kUpperWaves)
 // offsets.
 //
 // With the decomposition above, a workgroup takes one channel tile and, in
-// turn, each drift block of its chunk and each group of QG = 64/M consecutive
-// level-16 offsets q0..q0+QG-1: phase 1 computes R_q[k] (q in the group, every
+// turn, each drift block of its chunk and each group (or, split, one group) of
+// QG = 64/M consecutive level-16 offsets q0..q0+QG-1: phase 1 computes R_q[k] (q in the group, every
 // time block k: QG*M = 64 rows) into LDS, phase 2 runs the M-row trees of
 // those offsets from LDS and folds the keys. Nothing goes through HBM but the
 // input. Offsets of a group differ by at most QG-1, so time block k's trees
