@@ -264,6 +264,22 @@ def test_staged_split_groups_is_bit_identical(gpu, oracle, T, G, mode):
         assert got and len(got ^ want) <= 1
 
 
+@pytest.mark.parametrize("T", [16, 32, 64, 128, 256])
+def test_split_shape_only_where_compiled(gpu, T):
+    # (4, G, 1) is the split staged kernel at 128 and 256 time steps; at 32
+    # and 64 there is one offset group, so no such shape (at 16 the same
+    # triple is a different kernel: the prefetching window mode).
+    dd = sc.Dedopplerer(T, 4096, -2.79e-6, 18.25)
+    if T in (32, 64):
+        with pytest.raises(sc.SeticoreError):
+            dd.set_tuning(4, 8, 1)
+    else:
+        dd.set_tuning(4, 8, 1)
+    for G in (3, 64):
+        with pytest.raises(sc.SeticoreError):
+            dd.set_tuning(4, G, 1)
+
+
 def test_device_stats_mode(gpu, oracle):
     T, N = 16, 1 << 18
     rng = np.random.default_rng(5)
