@@ -44,15 +44,19 @@ def _rfi_waterfall(torch, seed):
     return x
 
 
-@pytest.mark.parametrize("T_r", [32, 64])
+@pytest.mark.parametrize("T_r", [32, 64, 128, 256])
 @pytest.mark.parametrize("dtype", [sc.SC_F16, sc.SC_BF16])
 def test_half_storage_parity_longer_windows(gpu, oracle, dtype, T_r):
-    # The one-pass (32) and two-pass (64) kernels on half-width samples equal
-    # the oracle on the same samples widened to float32.
+    # The default one-pass kernels at 32..256 time steps (LDS-staged; at 256
+    # one workgroup per offset group) on half-width samples equal the oracle
+    # on the same samples widened to float32.
     import torch
-    N = 1 << 15
+    N = 1 << 15 if T_r <= 64 else 1 << 13
+    print("half T_r=%d dtype=%d N=%d" % (T_r, dtype, N), flush=True)
     x = torch.empty((T_r, N), dtype=torch.float32, device="cuda")
-    sc.gen_waterfall(x, T_r, seed=T_r, tones=[(9000.0, 0.7, 3.0), (20000.0, -0.4, 2.0)])
+    tones = ([(9000.0, 0.7, 3.0), (20000.0, -0.4, 2.0)] if T_r <= 64 else
+             [(2000.0, 0.7, 3.0), (5000.0, -0.4, 2.0)])
+    sc.gen_waterfall(x, T_r, seed=T_r, tones=tones)
     xn, _, _ = sc.normalize(x, dtype)
     widened = xn.float().cpu().numpy()
     dd = sc.Dedopplerer(T_r, N, FOFF, TSAMP)
