@@ -926,11 +926,11 @@ __global__ void __launch_bounds__(kWave * kUpperWaves)
 // offsets.
 //
 // With the decomposition above, a workgroup takes one channel tile and, in
-// turn, each drift block of its chunk and each group (or, split, one group) of
-// QG = 64/M consecutive level-16 offsets q0..q0+QG-1: phase 1 computes R_q[k] (q in the group, every
-// time block k: QG*M = 64 rows) into LDS, phase 2 runs the M-row trees of
-// those offsets from LDS and folds the keys. Nothing goes through HBM but the
-// input. Offsets of a group differ by at most QG-1, so time block k's trees
+// turn, each drift block of its chunk and each group (or, split, one group)
+// of QG = 64/M consecutive level-16 offsets q0..q0+QG-1: phase 1 computes
+// R_q[k] (q in the group, every time block k: QG*M = 64 rows) into LDS,
+// phase 2 runs the M-row trees of those offsets from LDS and folds the keys.
+// Nothing goes through HBM but the input. Offsets of a group differ by at most QG-1, so time block k's trees
 // are needed over only Wr + (QG-1)*k channels, which one wave's 240 valid
 // level-16 outputs cover for every k. Offset q of level 16 needs offset q >> 1
 // of level 8, q >> 2 of level 4 and q >> 3 of level 2, so a group's partial
