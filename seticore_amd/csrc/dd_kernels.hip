@@ -1020,6 +1020,31 @@ __device__ __forceinline__ void staged_load(const In* __restrict__ x, long strid
                                             int b, long base, float (&v)[16][Staged<M>::C]) {
   constexpr int C = Staged<M>::C;
   const int lane = threadIdx.x % kWave;
+  // Row t starts at channel base + t*b, monotonic in t: rows 0 and 15 bound
+  // the block, so one check covers all 16 rows and the source pointer just
+  // advances by stride + b per row (no per-row address products or branches).
+  const long g_lo = b >= 0 ? base : base + 15L * b;
+  const long g_hi = b >= 0 ? base + 15L * b : base;
+  if (g_lo >= 0 && g_hi + kWave * C <= N) {
+    const In* src = x + (long)(16 * k) * stride + base + lane * C;
+    const long step = stride + b;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      if constexpr (sizeof(In) == 4) {
+        typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+        const f4u w = *(const f4u*)src;
+        v[t][0] = w.x;
+        v[t][1] = w.y;
+        v[t][2] = w.z;
+        v[t][3] = w.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < C; ++j) v[t][j] = to_f32<In>(src[j]);
+      }
+      src += step;
+    }
+    return;
+  }
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const In* row = x + (long)(16 * k + t) * stride;
