@@ -180,6 +180,22 @@ __device__ __forceinline__ void fetch_window(float (&tmp)[T][L], const In* __res
   using Wn = Window<T, C, G>;
   static_assert(L == Wn::kLoads, "register staging size");
   const int lane = threadIdx.x;
+  // Row t starts at c_w + t*b0, monotonic in t: the first and last rows bound
+  // the window, so one check covers every row and the pointer advances by
+  // stride + b0 per row.
+  const int g_lo = b0 >= 0 ? c_w : c_w + (T - 1) * b0;
+  const int g_hi = b0 >= 0 ? c_w + (T - 1) * b0 : c_w;
+  if (g_lo >= 0 && g_hi + Wn::kRow <= N) {
+    const In* src = x + (long)row0 * stride + c_w + lane;
+    const long step = stride + b0;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+#pragma unroll
+      for (int i = 0; i < Wn::kLoads; ++i) tmp[t][i] = to_f32<In>(src[i * kWave]);
+      src += step;
+    }
+    return;
+  }
 #pragma unroll
   for (int t = 0; t < T; ++t) {
     const int g0 = c_w + t * b0;
