@@ -62,10 +62,10 @@ void default_tuning(int T, int* lanes, int* group, int* prefetch) {
   // Measured on MI355X: at 16 x 2^20 (132 drift blocks) C=4, G=4 with chunked
   // windows (mode 2) is the fastest T=16 shape; at 32..256 x 2^20 (66 blocks)
   // the LDS-staged one-pass kernel with 8, 16, 8, 32 drift blocks per
-  // workgroup (1.29, 3.02, 6.85, 24.3 ms against 2.25 ms for fused32_kernel at
+  // workgroup (1.29, 2.83, 5.46, 20.1 ms against 2.25 ms for fused32_kernel at
   // 32 and 4.25, 7.83, 16.5, 33.7 ms for the two-pass search); at 256 with one
-  // workgroup per offset group (22.6 ms against 23.4 for one workgroup over
-  // all four groups at 8 blocks; at 128 the split is slower, 6.36 vs 6.29).
+  // workgroup per offset group (20.1 ms against 21.9 for one workgroup over
+  // all four groups at 8 blocks; at 128 the split is slower, 5.58 vs 5.46).
   // Below 16 time steps: 8 channels per lane, 16 drift blocks per window.
   const bool staged = T >= 32 && T <= 256;
   *lanes = (T == 16 || staged) ? 4 : 8;
