@@ -32,6 +32,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Dedoppler Gpath-sums/s & % HBM roofline; learned vs plain at ≥99.5% hit recall"
 HBM_PEAK_GBS = 8000.0
+# Vector ALU issue of one MI355X without packed FP32: 256 CUs x 4 SIMDs x 16
+# lanes per clock at 2.4 GHz, in lane-operations per second (the tree is adds
+# and maxima on the VALU; packed FP32 forms are not used).
+VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9
+# The box's CPU share for one GPU: the CPU baseline never runs more threads.
+HOST_THREAD_SHARE = 16
 
 # Workload constants.
 NUM_CC = 64
@@ -96,6 +102,56 @@ def load_traffic():
         return None
     with open(path) as f:
         return json.load(f).get("hbm_bytes_per_launch")
+
+
+def alg_valu_ops_per_path_sum(T_r):
+    """Algorithmic vector operations per path sum: log2(T_r) adds (one per
+    tree level, taylor.h:53-54) and half a max (a three-input max folds two
+    path sums into the running best, dedoppler.cu:47-52)."""
+    return math.log2(T_r) + 0.5
+
+
+def host_description():
+    """nproc, the CPUs this process may run on, and lscpu's model name."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"nproc": os.cpu_count(), "affinity_cpus": affinity, "lscpu_model": model}
+
+
+def hits_vs_oracle(gpu_hits, ref, data_cc, snr_threshold):
+    """Hit-set difference of a device-statistics search against the oracle's
+    (reference-order) search of the same coarse channel: hits keyed by start
+    channel; a difference is allowed only for a window whose largest top sum
+    lies between the reference's float threshold and the FP64 one
+    (dedoppler.cu:273-303). Common hits must agree on drift and path sum."""
+    from oracle import oracle
+    med64, _, std64 = oracle.stats_fp64(data_cc)
+    cands = [float(np.float32(snr_threshold * ref.std + ref.median))]
+    for f in (1 - 1e-6, 1 + 1e-6):
+        for m in (med64, ref.median):
+            cands.append(float(np.float32(snr_threshold * std64 * f + m)))
+    lo, hi = min(cands), max(cands)
+    got = {h.index: h for h in gpu_hits}
+    want = {h["index"]: h for h in ref.hits}
+    diffs = sorted(set(got) ^ set(want))
+    between = all(lo <= float(ref.top_sums[i]) <= hi for i in diffs)
+    common = set(got) & set(want)
+    agree = all((got[i].drift_steps, got[i].power) == (want[i]["drift_steps"], want[i]["power"])
+                for i in common)
+    return {"same": len(common), "only_gpu": len(set(got) - set(want)),
+            "only_oracle": len(set(want) - set(got)), "all_diffs_between_thresholds": between,
+            "common_hits_identical": agree}
 
 
 def main():
@@ -211,7 +267,10 @@ def main():
             print("batched hits differ from the per-call searches", file=sys.stderr)
             sys.exit(1)
         mode = "one batched graph launch over the coarse channels (hits equal to per-call searches)"
+        batch_hits_cc = {cc: batch.hits(cc) for cc in range(num_cc)}
         del batch
+    if args.loop:
+        batch_hits_cc = {cc: [h for h in hits if h.coarse_channel == cc] for cc in range(num_cc)}
     # Kernel duration, measured live with HIP events on the search stream
     # around the path-sum kernel (one extra pass, launched eagerly).
     dd.set_timing(True)
@@ -225,12 +284,22 @@ def main():
     value = total_ps / (elapsed / args.steps) / 1e9
 
     # Roofline of the dominant kernel (tree + top-path tracking over all drift
-    # blocks, one coarse channel per launch): algorithmic bytes per launch =
-    # D*T*N*4 (one streaming pass per drift block) + T*N*4 (column sums)
-    # + 12*N (top sums, blocks, offsets).
-    alg_bytes = nblocks * T * N * 4 + T * N * 4 + 12 * N
+    # blocks, one coarse channel per launch). It is bound by vector ALU issue:
+    # the input is read from HBM about once per launch and the drift blocks
+    # are served from LDS and L2. achieved = algorithmic VALU lane-operations
+    # per launch (path sums x (log2(T_r) adds + 1/2 max)) / launch time, peak
+    # = 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz. Beside it: the HBM traffic
+    # the counters measured per launch (profiles/) as a fraction of 8 TB/s,
+    # and the reference-dataflow-equivalent byte rate (D*T*N*4 + T*N*4 + 12*N
+    # bytes per launch: the reference streams the waterfall once per drift
+    # block, dedoppler.cu:214-227), which is not a fraction of any roof.
     avg_ms = kernel_ms / max(1, launches)
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    ops_per_ps = alg_valu_ops_per_path_sum(T)
+    alg_ops = per_cc * ops_per_ps
+    achieved_tops = alg_ops / (avg_ms * 1e-3) / 1e12
+    peak_tops = VALU_PEAK_OPS / 1e12
+    ref_dataflow_bytes = nblocks * T * N * 4 + T * N * 4 + 12 * N
+    traffic = load_traffic() if (T, MAX_DRIFT) == (16, 10.0) else None
 
     line = {
         "metric": METRIC,
@@ -255,34 +324,52 @@ def main():
             "per_call_loop_gpath_per_s": round(total_ps / (loop_elapsed / args.steps) / 1e9, 3),
         },
         "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic() if (T, MAX_DRIFT) == (16, 10.0) else None,
-            "kernel": "fused_tree_top_kernel" if T <= 16 else "column sums + staged_tree_kernel",
+            "bound": "valu",
+            "achieved": round(achieved_tops, 3),
+            "peak": round(peak_tops, 3),
+            "unit": "TOP/s (VALU lane-operations, non-packed)",
+            "frac": round(achieved_tops / peak_tops, 4),
+            "traffic": traffic,
+            "kernel": "fused_tree_top_kernel" if T <= 16 else "staged_tree_kernel",
             "kernel_ms_per_launch": round(avg_ms, 4),
-            "alg_bytes_per_launch": alg_bytes,
+            "path_sums_per_launch": per_cc,
+            "alg_ops_per_path_sum": ops_per_ps,
+            "alg_ops_per_launch": alg_ops,
+            "hbm_traffic_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+            "hbm_traffic_frac": (round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                 if traffic else None),
+            "reference_dataflow_equiv_gbs": round(ref_dataflow_bytes / (avg_ms * 1e-3) / 1e9, 1),
+            "reference_dataflow_bytes_per_launch": ref_dataflow_bytes,
         },
     }
 
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        host = host_description()
+        avail = host["affinity_cpus"] or host["nproc"] or 1
+        threads = max(1, min(HOST_THREAD_SHARE, avail))
         k = max(1, min(args.cpu_sample_cc, num_cc))
         total_dt = 0.0
+        diff = {"same": 0, "only_gpu": 0, "only_oracle": 0,
+                "all_diffs_between_thresholds": True, "common_hits_identical": True}
         for cc in range(k):
             sample = data[cc].cpu().numpy()
-            dt, _ = cpu_baseline(sample, threads)
+            dt, ref = cpu_baseline(sample, threads)
             total_dt += dt
+            d = hits_vs_oracle(batch_hits_cc[cc], ref, sample, SNR_THRESHOLD)
+            for key_ in ("same", "only_gpu", "only_oracle"):
+                diff[key_] += d[key_]
+            for key_ in ("all_diffs_between_thresholds", "common_hits_identical"):
+                diff[key_] = diff[key_] and d[key_]
         line["cpu_baseline"] = {
             "value": round(per_cc * k / total_dt / 1e9, 4),
             "unit": "Gpath-sums/s",
             "cores": threads,
             "kind": "port",
             "sample": "%d coarse channel(s) of the same workload (%d x 2^20, %d drift "
-                      "blocks), full search incl. host statistics, %d threads"
-                      % (k, T, nblocks, threads),
+                      "blocks), full search incl. host statistics, %d threads (the box's "
+                      "CPU share for one GPU is %d)" % (k, T, nblocks, threads, HOST_THREAD_SHARE),
+            "host": host,
+            "hits_vs_oracle": diff,
         }
     print(json.dumps(line), flush=True)
 

@@ -29,7 +29,8 @@ typedef enum {
   SC_ERR_HIP = 2,              /* a HIP runtime error (the reference exits with code 2) */
   SC_ERR_OUT_OF_MEMORY = 3,
   SC_ERR_UNSUPPORTED = 4,      /* a shape this build has no kernel for */
-  SC_ERR_IO = 5                /* output file errors (writers) */
+  SC_ERR_IO = 5,               /* output file errors (writers) */
+  SC_ERR_INTERNAL = 6          /* a consistency check inside the library failed */
 } sc_status;
 
 /* Storage type of the power samples handed to sc_dd_search. The reference
@@ -140,7 +141,9 @@ sc_status sc_dd_kernel_time(const sc_dd* dd, double* total_ms, long* searches);
  * Up to `capacity` of them are copied to `hits`; *num_hits receives the total,
  * which may exceed capacity (then fetch the rest with sc_dd_copy_hits).
  * stream: a hipStream_t (NULL = default stream). The call returns after the
- * search is complete, like the reference's blocking copies (dedoppler.cu:232). */
+ * search is complete, like the reference's blocking copies (dedoppler.cu:232).
+ * Samples must be finite: input with NaN or infinite samples (detected through
+ * the column sums) fails with SC_ERR_INVALID_ARGUMENT. */
 sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_timesteps,
                        int num_channels, long row_stride, const sc_meta* meta,
                        int beam, int coarse_channel, double max_drift,

@@ -37,6 +37,7 @@ SC_ERR_HIP = 2
 SC_ERR_OUT_OF_MEMORY = 3
 SC_ERR_UNSUPPORTED = 4
 SC_ERR_IO = 5
+SC_ERR_INTERNAL = 6
 
 SC_F32, SC_F16, SC_BF16 = 0, 1, 2
 SC_STATS_HOST_EXACT, SC_STATS_DEVICE = 0, 1

@@ -15,8 +15,14 @@ import ctypes
 import json
 import os
 
-from . import (SC_F32, SC_OK, SC_STATS_DEVICE, Dedopplerer, FilterbankMetadata, ScHit, ScMeta,
-               SeticoreError, _stream_ptr, check, gen_waterfall, lib, round_up_to_power_of_two)
+from . import (SC_BF16, SC_F16, SC_F32, SC_OK, SC_STATS_DEVICE, Dedopplerer, FilterbankMetadata,
+               ScHit, ScMeta, SeticoreError, _stream_ptr, check, gen_waterfall, lib,
+               round_up_to_power_of_two)
+
+
+def _torch_mod():
+    import torch
+    return torch
 
 
 class ScInstance(ctypes.Structure):
@@ -63,13 +69,31 @@ def _lib():
 
 def instance(data, num_timesteps, foff, tsamp, max_drift, min_drift, snr_threshold, beam=-1,
              coarse_channel=0, dtype=SC_F32, metadata=None):
-    """An ScInstance for a CUDA tensor [T_r][N] (kept alive by the caller)."""
+    """An ScInstance for a CUDA tensor [T_r][N], T_r = roundUpToPowerOfTwo(
+    num_timesteps), of the sample type `dtype`, unit channel stride (the checks
+    FilterbankBuffer makes: the batch graph reads T_r rows of N samples through
+    the raw pointer). The instance holds a reference to the tensor, and a
+    Batch holds its instances', so the memory outlives the captured graph."""
+    torch = _torch_mod()
+    if data.dim() != 2:
+        raise ValueError("instance data must be a 2-D [T_r][N] tensor")
     T_r, N = data.shape
+    if T_r != round_up_to_power_of_two(int(num_timesteps)):
+        raise ValueError("instance data has %d rows; %d time steps need %d" %
+                         (T_r, num_timesteps, round_up_to_power_of_two(int(num_timesteps))))
+    want = {SC_F32: torch.float32, SC_F16: torch.float16, SC_BF16: torch.bfloat16}.get(dtype)
+    if want is None or data.dtype != want:
+        raise ValueError("instance data dtype %s does not match sample type %r" %
+                         (data.dtype, dtype))
+    if not data.is_cuda or data.stride(1) != 1:
+        raise ValueError("instance data must be a CUDA tensor with unit channel stride")
     meta = (metadata or FilterbankMetadata(coarse_channel_size=N, foff=foff))._sc_meta()
-    return ScInstance(data.data_ptr(), int(dtype), int(num_timesteps), int(N),
+    inst = ScInstance(data.data_ptr(), int(dtype), int(num_timesteps), int(N),
                       int(data.stride(0)), float(foff), float(tsamp), meta, int(beam),
                       int(coarse_channel), float(max_drift), float(min_drift),
                       float(snr_threshold))
+    inst._source = data
+    return inst
 
 
 def multibeam_instances(multibeam, num_timesteps, coarse_channel_size, foff, tsamp, max_drift,
@@ -98,6 +122,9 @@ class Batch:
 
     def __init__(self, instances, tuning=()):
         L = _lib()
+        # The graph reads the instances' memory through raw pointers: keep the
+        # source tensors alive as long as the batch.
+        self._sources = [getattr(i, "_source", None) for i in instances]
         self._inst = (ScInstance * len(instances))(*instances)
         self._tuning = (ScTuning * max(1, len(tuning)))(*tuning)
         h = ctypes.c_void_p()

@@ -56,6 +56,9 @@ int round_up_pow2(int n) {
   return r;
 }
 
+}  // namespace
+
+namespace sc {
 // Default fused-kernel shape per T_r: channels per lane and drift blocks per
 // LDS window.
 void default_tuning(int T, int* lanes, int* group, int* prefetch) {
@@ -73,7 +76,7 @@ void default_tuning(int T, int* lanes, int* group, int* prefetch) {
   *prefetch = T == 16 ? 2 : T == 32 ? sc::kStaged32 : T == 256 ? 1 : 0;
 }
 
-}  // namespace
+}  // namespace sc
 
 struct sc_dd {
   int num_timesteps = 0;
@@ -220,20 +223,10 @@ void release(sc_dd* dd) {
   if (dd->ev_join) (void)hipEventDestroy(dd->ev_join);
 }
 
-struct DriftRange {
-  int bmin, bmax, window;
-  double nmd;
-};
+using sc::DriftRange;
 
-// dedoppler.cu:195-198, 274.
 DriftRange drift_range(const sc_dd* dd, double max_drift) {
-  DriftRange r;
-  double diagonal = dd->resolution * dd->drift_timesteps;
-  r.nmd = max_drift / std::fabs(diagonal);
-  r.bmin = (int)std::floor(-r.nmd);
-  r.bmax = (int)std::floor(r.nmd);
-  r.window = (int)(2 * std::ceil(r.nmd * dd->drift_timesteps));
-  return r;
+  return sc::drift_range_of(dd->resolution, dd->drift_timesteps, max_drift);
 }
 
 sc_status ensure_tree_scratch(sc_dd* dd) {
@@ -416,7 +409,7 @@ sc_status sc_dd_create(int num_timesteps, int num_channels, double foff_mhz, dou
   dd->drift_timesteps = rounded - 1;
   // dedoppler.cu:87
   dd->resolution = 1e6 * foff_mhz / (dd->drift_timesteps * tsamp_s);
-  default_tuning(rounded, &dd->lanes, &dd->group, &dd->prefetch);
+  sc::default_tuning(rounded, &dd->lanes, &dd->group, &dd->prefetch);
   if (rounded <= 256 && !sc::fused_supported(rounded, dd->lanes, dd->group, dd->prefetch)) {
     // Every T_r up to 256 has a compiled one-pass default; never fall back
     // to the staged global-memory path silently.
@@ -633,7 +626,7 @@ sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_thresho
            "candidate offsets");
   SC_HIP(hipMemcpyAsync(h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost, stream),
          "copy candidates");
-  SC_HIP(hipMemcpyAsync(h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
+  SC_HIP(hipMemcpyAsync(h_stats, dd->stats_out, 4 * sizeof(float), hipMemcpyDeviceToHost,
                         stream), "copy stats");
   return SC_OK;
 }
@@ -683,16 +676,22 @@ sc_status search_graph(sc_dd* dd, const void* data, int dtype, long row_stride,
   return SC_OK;
 }
 
-// Hits from the per-window candidates (dedoppler.cu:305-317).
-void device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
-                 int coarse_channel, double min_drift, const int4* h_cand = nullptr,
-                 const float* h_stats = nullptr, std::vector<sc_hit>* out = nullptr,
-                 int num_timesteps = 0) {
+// Hits from the per-window candidates (dedoppler.cu:305-317). h_stats holds
+// {median, std, mean, number of non-finite column sums}. Fails on input with
+// NaN or infinite samples, and if a candidate's offset could not be recovered.
+sc_status device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
+                      int coarse_channel, double min_drift, const int4* h_cand = nullptr,
+                      const float* h_stats = nullptr, std::vector<sc_hit>* out = nullptr,
+                      int num_timesteps = 0) {
   if (!h_cand) h_cand = dd->h_cand;
   if (!h_stats) h_stats = dd->h_stats;
   if (!out) out = &dd->hits;
   if (num_timesteps <= 0) num_timesteps = dd->num_timesteps;
   const int N = dd->num_channels;
+  if (h_stats[3] != 0.0f)
+    return fail(SC_ERR_INVALID_ARGUMENT,
+                "input has NaN or infinite samples (" + std::to_string((long)h_stats[3]) +
+                    " non-finite column sums)");
   const float median = h_stats[0], std_dev = h_stats[1];
   dd->last_median = median;
   dd->last_std = std_dev;
@@ -700,6 +699,9 @@ void device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
   for (int i = 0; i < nwin; ++i) {
     const int4 c = h_cand[i];
     if (c.x < 0) continue;
+    if (c.w < 0)
+      return fail(SC_ERR_INTERNAL, "no path offset reproduces the top sum at channel " +
+                                       std::to_string(c.x));
     float cand_sum;
     std::memcpy(&cand_sum, &c.y, sizeof(float));
     int drift_bins = c.z * dd->drift_timesteps + c.w;
@@ -709,6 +711,7 @@ void device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
       out->push_back(make_hit(meta, c.x, drift_bins, drift_rate, snr, beam, coarse_channel,
                               num_timesteps, cand_sum));
   }
+  return SC_OK;
 }
 
 }  // namespace
@@ -756,6 +759,17 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
     SC_HIP(hipMemcpyAsync(dd->h_top_off, dd->top_off, N * sizeof(int), hipMemcpyDeviceToHost,
                           stream), "copy top offsets");
     SC_HIP(hipStreamSynchronize(stream), "search");
+    // NaN or infinite samples make their column's sum non-finite (bit test:
+    // exponent all ones). The key arithmetic assumes finite samples.
+    long bad = 0;
+    for (int c = 0; c < N; ++c) {
+      uint32_t u;
+      std::memcpy(&u, &dd->h_colsum[c], sizeof(u));
+      bad += (u & 0x7F800000u) == 0x7F800000u;
+    }
+    if (bad)
+      return fail(SC_ERR_INVALID_ARGUMENT, "input has NaN or infinite samples (" +
+                                               std::to_string(bad) + " non-finite column sums)");
     float median, std_dev;
     host_exact_stats(dd->h_colsum, N, &median, &std_dev);
     dd->last_median = median;
@@ -800,7 +814,8 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
       if (s != SC_OK) return s;
     }
     SC_HIP(hipStreamSynchronize(stream), "search");
-    device_hits(dd, r, meta, beam, coarse_channel, min_drift);
+    s = device_hits(dd, r, meta, beam, coarse_channel, min_drift);
+    if (s != SC_OK) return s;
   }
   if (dd->timing) {
     float ms = 0.0f;
@@ -889,12 +904,13 @@ sc_status sc_dd_search_proposed(sc_dd* dd, sc_pn* pn, const void* data, int dtyp
   if (dd->timing) SC_HIP(hipEventRecord(dd->ev_stop, stream), "timing event");
   SC_HIP(hipMemcpyAsync(dd->h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost,
                         stream), "copy candidates");
-  SC_HIP(hipMemcpyAsync(dd->h_stats, dd->stats_out, 2 * sizeof(float), hipMemcpyDeviceToHost,
+  SC_HIP(hipMemcpyAsync(dd->h_stats, dd->stats_out, 4 * sizeof(float), hipMemcpyDeviceToHost,
                         stream), "copy stats");
   SC_HIP(hipMemcpyAsync(dd->h_gate, dd->gate_count, 2 * sizeof(int), hipMemcpyDeviceToHost,
                         stream), "copy tile count");
   SC_HIP(hipStreamSynchronize(stream), "gated search");
-  device_hits(dd, r, meta, beam, coarse_channel, min_drift);
+  s = device_hits(dd, r, meta, beam, coarse_channel, min_drift);
+  if (s != SC_OK) return s;
   if (dd->timing) {
     float ms = 0.0f;
     SC_HIP(hipEventElapsedTime(&ms, dd->ev_start, dd->ev_stop), "timing");
@@ -1111,6 +1127,9 @@ extern "C" sc_status sc_condition_coarse_channel(float* data, int num_timesteps,
   if (num_timesteps < 0 || rounded_timesteps < num_timesteps || num_channels <= 0 ||
       row_stride < num_channels)
     return fail(SC_ERR_INVALID_ARGUMENT, "bad shape");
+  if (has_dc_spike && num_channels < 3)
+    return fail(SC_ERR_INVALID_ARGUMENT,
+                "coarse channels of fewer than 3 channels cannot take the DC-spike fix");
   SC_HIP(sc::launch_condition(data, num_timesteps, rounded_timesteps, num_channels, row_stride,
                               has_dc_spike, (hipStream_t)stream),
          "condition");
@@ -1146,7 +1165,7 @@ struct sc_batch {
   std::vector<sc_dd*> shapes;   // handles; a shape's alternates are consecutive
   std::vector<int> alternates;  // per handle: handles of its shape (1 or 2)
   int4* h_cand = nullptr;
-  float* h_stats = nullptr;  // 2 per instance
+  float* h_stats = nullptr;  // 4 per instance
   hipStream_t capture_stream = nullptr;
   hipStream_t tail_stream = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1241,7 +1260,7 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
   }
   if (s == SC_OK) s = host_alloc((void**)&b->h_cand, total_windows * sizeof(int4), "candidates");
   if (s == SC_OK)
-    s = host_alloc((void**)&b->h_stats, 2 * sizeof(float) * (size_t)count, "statistics");
+    s = host_alloc((void**)&b->h_stats, 4 * sizeof(float) * (size_t)count, "statistics");
   if (s == SC_OK) {
     hipError_t e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->tail_stream, hipStreamNonBlocking);
@@ -1275,7 +1294,7 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
       if (e != hipSuccess) break;
       s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, bi.in.data, bi.in.dtype,
                               bi.in.row_stride, b->tail_stream, b->h_cand + bi.cand_offset,
-                              b->h_stats + 2 * i);
+                              b->h_stats + 4 * i);
       if (s != SC_OK) break;
       e = hipEventRecord(tail, b->tail_stream);
       last_tail[bi.shape] = tail;
@@ -1317,9 +1336,12 @@ sc_status sc_batch_run(sc_batch* b, void* stream_ptr) {
   for (size_t i = 0; i < b->inst.size(); ++i) {
     sc_batch::Instance& bi = b->inst[i];
     bi.hits.clear();
-    device_hits(b->shapes[bi.shape], bi.r, &bi.in.meta, bi.in.beam, bi.in.coarse_channel,
-                bi.in.min_drift, b->h_cand + bi.cand_offset, b->h_stats + 2 * i, &bi.hits,
-                bi.in.num_timesteps);
+    const sc_status s =
+        device_hits(b->shapes[bi.shape], bi.r, &bi.in.meta, bi.in.beam, bi.in.coarse_channel,
+                    bi.in.min_drift, b->h_cand + bi.cand_offset, b->h_stats + 4 * i, &bi.hits,
+                    bi.in.num_timesteps);
+    if (s != SC_OK)
+      return fail(s, "instance " + std::to_string(i) + ": " + sc::last_error());
   }
   return SC_OK;
 }

@@ -3,9 +3,31 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 namespace sc {
+
+// Drift-block range and merge window of a search (dedoppler.cu:195-198, 274):
+// nmd = max_drift / |resolution * drift_timesteps|, blocks floor(-nmd) ..
+// floor(nmd), window 2 * ceil(nmd * drift_timesteps).
+struct DriftRange {
+  int bmin, bmax, window;
+  double nmd;
+};
+inline DriftRange drift_range_of(double resolution, int drift_timesteps, double max_drift) {
+  DriftRange r;
+  const double diagonal = resolution * drift_timesteps;
+  r.nmd = max_drift / fabs(diagonal);
+  r.bmin = (int)floor(-r.nmd);
+  r.bmax = (int)floor(r.nmd);
+  r.window = (int)(2 * ceil(r.nmd * drift_timesteps));
+  return r;
+}
+
+// Default one-pass kernel shape for T_r (channels per lane, drift blocks per
+// LDS window or workgroup, window mode / kernel selector), as measured best.
+void default_tuning(int T, int* lanes, int* group, int* prefetch);
 
 // Per-channel search state: the bits of an FP64 key whose value orders
 // (top path sum, -code) lexicographically - the float sum widened to double

@@ -547,7 +547,9 @@ __global__ void __launch_bounds__(kWave)
     if (p_best < 0 && mine) p_best = p0 + __ffsll((long long)mine) - 1;
   }
   if (lane == 0 && w < nwin && r.x >= 0) {
-    r.w = p_best < 0 ? 0 : p_best;
+    // -1: no offset reproduces the recorded sum (the tree kernels and this
+    // re-addition disagree on an addition order); the host reports an error.
+    r.w = p_best;
     cand[w] = r;
     top_off[r.x] = r.w;
   }
@@ -1382,8 +1384,11 @@ constexpr int kStatsBlocks = 256;
 constexpr int kStatsThreads = 256;
 constexpr int kL1Bits = 11, kL2Bits = 11, kL3Bits = 10;
 constexpr int kL1 = 1 << kL1Bits, kL2 = 1 << kL2Bits, kL3 = 1 << kL3Bits;
-// Histogram words: level 1, then level 2 and level 3 for each of the 3 ranks.
+// Histogram words: level 1, then level 2 and level 3 for each of the 3 ranks,
+// then one word counting column sums that are not finite (NaN or infinite
+// samples; the search rejects such input).
 constexpr int kHistWords = kL1 + 3 * kL2 + 3 * kL3;
+constexpr int kNonFiniteWord = kHistWords;
 
 struct StatsState {
   unsigned key[3];    // exact order keys of ranks mid, first, last
@@ -1393,7 +1398,7 @@ struct StatsState {
   unsigned part_n[kStatsBlocks];
 };
 
-size_t stats_hist_words() { return (size_t)kHistWords; }
+size_t stats_hist_words() { return (size_t)kHistWords + 1; }
 
 __device__ __forceinline__ int rank_first(int N) { return (int)ceil(0.05 * N); }
 __device__ __forceinline__ int rank_last(int N) { return (int)floor(0.95 * N); }
@@ -1481,8 +1486,15 @@ __global__ void __launch_bounds__(kStatsThreads)
   __shared__ unsigned h[kL1];
   for (int i = threadIdx.x; i < kL1; i += blockDim.x) h[i] = 0;
   __syncthreads();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
-    atomicAdd(&h[ord_key(v[i]) >> (32 - kL1Bits)], 1u);
+  unsigned bad = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const float x = v[i];
+    // Exponent bits all ones: NaN or infinity (an integer test, which the
+    // no-NaN compile mode cannot fold away).
+    bad += (__float_as_uint(x) & 0x7F800000u) == 0x7F800000u;
+    atomicAdd(&h[ord_key(x) >> (32 - kL1Bits)], 1u);
+  }
+  if (bad) atomicAdd(&hist[kNonFiniteWord], bad);
   __syncthreads();
   for (int i = threadIdx.x; i < kL1; i += blockDim.x)
     if (h[i]) atomicAdd(&hist[i], h[i]);
@@ -1610,7 +1622,8 @@ __global__ void __launch_bounds__(kStatsThreads)
 }
 
 __global__ void __launch_bounds__(kStatsThreads)
-    stats_final_kernel(const StatsState* st, int N, float* __restrict__ stats_out) {
+    stats_final_kernel(const StatsState* st, int N, const unsigned* __restrict__ hist,
+                       float* __restrict__ stats_out) {
   __shared__ double shd[kStatsThreads];
   const int tid = threadIdx.x;
   // Fixed-order tree reduction of the per-workgroup partials.
@@ -1648,7 +1661,7 @@ __global__ void __launch_bounds__(kStatsThreads)
   stats_out[0] = median;
   stats_out[1] = (float)sqrt(fmax(acc, 0.0) / (double)cnt);
   stats_out[2] = m;
-  stats_out[3] = 0.0f;
+  stats_out[3] = (float)hist[kNonFiniteWord];  // non-finite column sums (0 for valid input)
 }
 
 // ---------------------------------------------------------------------------
@@ -2339,7 +2352,7 @@ hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsign
   hipLaunchKernelGGL(stats_pass2_kernel, grid, block, 0, stream, colsum, N, hist);
   hipLaunchKernelGGL(stats_pass3_kernel, grid, block, 0, stream, colsum, N, hist);
   hipLaunchKernelGGL(stats_pass4_kernel, grid, block, 0, stream, colsum, N, hist, st);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(1), block, 0, stream, st, N, stats_out);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), block, 0, stream, st, N, hist, stats_out);
   return hipGetLastError();
 }
 

@@ -200,6 +200,11 @@ sc_status sc_fil_set_coarse_channel_size(sc_fil* h, long coarse_channel_size) {
   if (!h) return fail(SC_ERR_INVALID_ARGUMENT, "null handle");
   if (coarse_channel_size <= 0 || h->meta.num_channels % coarse_channel_size != 0)
     return fail(SC_ERR_INVALID_ARGUMENT, "coarse channel size must divide nchans");
+  // The DC fix averages the centre channel's two neighbours (h5_reader.cpp:202-209),
+  // which a coarse channel needs at least three channels to have.
+  if (h->meta.has_dc_spike && coarse_channel_size < 3)
+    return fail(SC_ERR_INVALID_ARGUMENT,
+                "coarse channels of fewer than 3 channels cannot take the DC-spike fix");
   h->meta.coarse_channel_size = coarse_channel_size;
   h->meta.num_coarse_channels = h->meta.num_channels / coarse_channel_size;
   return SC_OK;
@@ -223,6 +228,9 @@ sc_status sc_fil_read_coarse_channel(const sc_fil* h, int i, int rounded_timeste
   if (rounded_timesteps < m.num_timesteps)
     return fail(SC_ERR_INVALID_ARGUMENT, "rounded_timesteps below the file's time steps");
   const long ccs = m.coarse_channel_size;
+  if (m.has_dc_spike && ccs < 3)
+    return fail(SC_ERR_INVALID_ARGUMENT,
+                "coarse channels of fewer than 3 channels cannot take the DC-spike fix");
   for (long t = 0; t < m.num_timesteps; ++t) {
     const long off = h->data_start + 4 * (t * (long)m.num_channels + i * ccs);
     if (std::fseek(h->file, off, SEEK_SET) != 0 ||

@@ -197,3 +197,24 @@ def test_hit_to_string():
     h.drift_steps = -1
     h.drift_rate = 0.0102
     assert h.toString().endswith("(-1 bin)")
+
+
+def test_batch_instance_checks_its_tensor():
+    # The batch graph reads T_r rows of N samples through the raw pointer, so
+    # the instance must be [roundUpToPowerOfTwo(T)][N] of the stated sample
+    # type, with unit channel stride, on the GPU (checked before any GPU call).
+    import torch
+    from seticore_amd import batch as B
+    args = (-2.79e-6, 18.25, 2.0, 0.0, 10.0)
+    with pytest.raises(ValueError, match="rows"):
+        B.instance(torch.zeros((15, 8192)), 15, *args)       # 15 steps need 16 rows
+    with pytest.raises(ValueError, match="rows"):
+        B.instance(torch.zeros((16, 8192)), 32, *args)
+    with pytest.raises(ValueError, match="dtype"):
+        B.instance(torch.zeros((16, 8192), dtype=torch.float16), 16, *args)  # default SC_F32
+    with pytest.raises(ValueError, match="dtype"):
+        B.instance(torch.zeros((16, 8192)), 16, *args, dtype=sc.SC_BF16)
+    with pytest.raises(ValueError, match="CUDA"):
+        B.instance(torch.zeros((16, 8192)), 16, *args)      # host memory
+    with pytest.raises(ValueError, match="2-D"):
+        B.instance(torch.zeros((2, 16, 8192)), 16, *args)

@@ -97,3 +97,67 @@ def test_find_events_on_a_synthetic_cadence(gpu, tmp_path):
     # The same signal in an "off" scan: no event.
     files = _cadence(tmp_path, "b", signal_in_off=True)
     assert sc.findEvents(files, str(tmp_path / "b.events"), 1.0, 12.0, 12.0) == 0
+
+
+def _cadence_several(tmp_path):
+    """Six 16 x 2^20 ABACAD scans, 300 s apart, with four drifting signals:
+    one also in "off" scan 3, two only in the "on" scans, one only in the
+    first scan. Returns (file names, [(tstart, data)] for the oracle)."""
+    T, N = 16, 1 << 20
+    tsamp = 18.253611
+    signals = [  # (start channel, channels per step, amplitude, scans)
+        (1000000, 0.5, 12.0, (0, 2, 3, 4)),
+        (1020000, -0.4, 12.0, (0, 2, 4)),
+        (1040000, 0.5, 12.0, (0, 2, 4)),
+        (1046000, 0.5, 12.0, (0,)),
+    ]
+    files, scans = [], []
+    for i in range(6):
+        rng = np.random.default_rng(300 + i)
+        data = rng.exponential(1.0, size=(T, N)).astype(np.float32)
+        for c0, slope, amp, which in signals:
+            if i in which:
+                for t in range(T):
+                    data[t, int(round(c0 + slope * (i * 300.0 / tsamp + t)))] += amp
+        on = i % 2 == 0
+        tstart = 57650.0 + i * 300.0 / 86400.0
+        p = tmp_path / ("c%d.fil" % i)
+        write_fil(p, data)
+        raw = p.read_bytes()
+        k = raw.index(b"tstart") + len(b"tstart")
+        raw = raw[:k] + np.float64(tstart).tobytes() + raw[k + 8:]
+        k = raw.index(b"VOYAGER1")
+        raw = raw[:k] + (b"TARGET01" if on else b"OFFSRC%02d" % i) + raw[k + 8:]
+        p.write_bytes(raw)
+        files.append(str(p))
+        scans.append((tstart, data))
+    return files, scans
+
+
+@pytest.mark.gpu
+def test_find_events_matches_the_oracle(gpu, oracle, tmp_path):
+    # sc_find_events against the restatement of find_events.cpp:124-217
+    # (oracle/events_oracle.py) fed by the oracle's search: the same events,
+    # each with the same hit (index, drift, SNR bit for bit: both use the
+    # reference's host statistics) or none per scan. Of the four candidates
+    # two are rejected, one by an "off" scan's hit in its range, one because
+    # the later "on" scans have no hit at or after its window.
+    from oracle import events_oracle
+    files, scans = _cadence_several(tmp_path)
+    out = tmp_path / "c.events"
+    n = sc.findEvents(files, str(out), 1.0, 12.0, 12.0)
+    got = hitfile.read_events(str(out))
+    want, outcomes = events_oracle.find_events(scans, 1 << 20, True, -2.7939677e-06, 18.253611,
+                                               1.0, 12.0, 12.0)
+    assert [k for k, _ in outcomes[0]] == [1000000, 1020000, 1040000, 1046000]
+    assert [v for _, v in outcomes[0]] == ["range", "event", "event", "on"]
+    assert n == len(got) == len(want[0]) == 2
+    for ev_got, ev_want in zip(got, want[0]):
+        assert len(ev_got) == len(ev_want) == 6
+        for g, w in zip(ev_got, ev_want):
+            if w is None:
+                assert g["signal"] is None
+            else:
+                s = g["signal"]
+                assert (s["index"], s["drift_steps"], s["snr"], s["power"]) == \
+                    (w["index"], w["drift_steps"], w["snr"], w["power"])

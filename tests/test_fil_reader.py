@@ -116,3 +116,24 @@ def test_errors_are_status_codes(tmp_path, case):
         p.write_bytes(raw[:i] + struct.pack("<i", 9) + raw[i + 4:])
     with pytest.raises(sc.SeticoreError):
         sc.FilReader(str(p))
+
+
+def test_small_coarse_channels_with_dc_spike_are_rejected(tmp_path):
+    # The DC fix reads the centre channel's two neighbours (h5_reader.cpp:202-209):
+    # a Green Bank file (DC spike inferred) cannot be cut into coarse channels
+    # of one or two channels; three channels work and fix channel 1.
+    T, N = 16, 1 << 20
+    data = np.arange(T * N, dtype=np.float32).reshape(T, N) % 7
+    p = tmp_path / "gbt.fil"
+    write_fil(p, data)
+    r = sc.FilReader(str(p))
+    assert r.has_dc_spike
+    for bad in (1, 2):
+        with pytest.raises(sc.SeticoreError):
+            r.set_coarse_channel_size(bad)
+    assert r.coarse_channel_size == N  # unchanged after a rejected size
+    r.set_coarse_channel_size(4)
+    got = r.read_coarse_channel(5, 16)
+    want = data[:, 20:24].copy()
+    want[:, 2] = (want[:, 1] + want[:, 3]) / np.float32(2.0)
+    np.testing.assert_array_equal(got, want)

@@ -436,6 +436,38 @@ def test_cpp_facade_known_answer(gpu, tmp_path):
     assert len(hits) == 1 and hits[0]["signal"]["drift_steps"] == 3
 
 
+def test_cpp_facade_incoherent_power_keeps_fields_with_hits(gpu):
+    """Dedopplerer::addIncoherentPower through the C++ façade re-sorts six hits
+    by drift steps (dedoppler.cu:144); every field must stay with its hit."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "tests", "cpp", "incoherent_order")
+    assert os.path.exists(exe), "build the C++ test with seticore_amd.build"
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert res.stdout.startswith("ok 6 hits")
+
+
+def test_non_finite_samples_are_rejected(gpu):
+    # Both statistics modes detect NaN/inf samples through the column sums.
+    T, N = 16, 8192
+    for mode in (sc.SC_STATS_HOST_EXACT, sc.SC_STATS_DEVICE):
+        for bad in (float("nan"), float("inf")):
+            data = _waterfall(T, N, 3)
+            data[5, 1234] = bad
+            dd = sc.Dedopplerer(T, N, -2.79e-6, 18.25)
+            dd.set_stats_mode(mode)
+            with pytest.raises(sc.SeticoreError) as e:
+                dd.search(sc.FilterbankBuffer(T, N, data=_to_gpu(data)),
+                          sc.FilterbankMetadata(coarse_channel_size=N), -1, 0, 2.0, 0.0, 6.0, [])
+            assert "NaN or infinite" in str(e.value)
+            # The same handle still searches finite input.
+            data[5, 1234] = 1.0
+            dd.search(sc.FilterbankBuffer(T, N, data=_to_gpu(data)),
+                      sc.FilterbankMetadata(coarse_channel_size=N), -1, 0, 2.0, 0.0, 6.0, [])
+
+
 # --- loader conditioning (h5_reader.cpp:195-209) and .fil input -------------
 def test_fil_load_and_condition_on_device(gpu, oracle, tmp_path):
     import torch
