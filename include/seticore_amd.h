@@ -150,6 +150,29 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
                        double min_drift, double snr_threshold, sc_hit* hits,
                        size_t capacity, size_t* num_hits, void* stream);
 
+/* Allocation-free search. No counterpart in the reference: it is for callers
+ * that cannot allocate memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). It runs the search of
+ * sc_dd_search with device statistics (SC_STATS_DEVICE) and the default
+ * kernel shapes, taking the Dedopplerer's constructor arguments (num_timesteps,
+ * foff, tsamp, as sc_dd_create) with the search's instead of a handle, and
+ * allocates nothing: every intermediate lives in one fixed device array of
+ * SC_STATIC_SCRATCH_BYTES per GPU, set up once with the library and held
+ * until the process ends. Limits: 4096 <= num_channels <=
+ * SC_STATIC_MAX_CHANNELS and at most 256 rounded time steps (larger inputs
+ * fail with SC_ERR_INVALID_ARGUMENT or SC_ERR_UNSUPPORTED). Calls that
+ * overlap on one GPU race, since they share that GPU's array. Up to `capacity`
+ * hits (ascending channel order) are written to `hits`; *num_hits receives the
+ * total. Returns when the search is complete. */
+#define SC_STATIC_MAX_CHANNELS (1L << 20)
+#define SC_STATIC_SCRATCH_BYTES 33620480L
+sc_status sc_dd_search_static(int num_timesteps, int rounded_timesteps, int num_channels,
+                              double foff_mhz, double tsamp_s, const void* data, int dtype,
+                              long row_stride, const sc_meta* meta, int beam,
+                              int coarse_channel, double max_drift, double min_drift,
+                              double snr_threshold, sc_hit* hits, size_t capacity,
+                              size_t* num_hits, void* stream);
+
 /* Copies hits of the last sc_dd_search (no counterpart: the reference appends
  * to a vector). */
 sc_status sc_dd_copy_hits(const sc_dd* dd, sc_hit* hits, size_t capacity,

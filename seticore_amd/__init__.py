@@ -133,6 +133,8 @@ def lib():
         "sc_dd_kernel_time": [vp, P(d), P(lng)],
         "sc_dd_search": [vp, vp, i, i, i, lng, P(ScMeta), i, i, d, d, d, P(ScHit), sz,
                          P(sz), vp],
+        "sc_dd_search_static": [i, i, i, d, d, vp, i, lng, P(ScMeta), i, i, d, d, d,
+                                P(ScHit), sz, P(sz), vp],
         "sc_dd_copy_hits": [vp, P(ScHit), sz, P(sz)],
         "sc_dd_copy_state": [vp, vp, vp, vp, vp, P(ctypes.c_float), P(ctypes.c_float)],
         "sc_dd_add_incoherent_power": [vp, vp, i, i, i, lng, P(ScHit), sz, vp],
@@ -463,6 +465,29 @@ def drift_parameters(num_timesteps, foff, tsamp, max_drift):
     res = 1e6 * foff / ((T - 1) * tsamp)
     nmd = max_drift / abs(res * (T - 1))
     return (int(math.floor(-nmd)), int(math.floor(nmd)), int(2 * math.ceil(nmd * (T - 1))), res)
+
+
+def search_static(input, num_timesteps, foff, tsamp, metadata, beam, coarse_channel, max_drift,
+                  min_drift, snr_threshold, stream=None):
+    """The allocation-free search (``sc_dd_search_static``): the device-statistics
+    search of :meth:`Dedopplerer.search` without a handle, its intermediates in
+    the library's fixed per-GPU array. ``input`` holds the rounded time steps;
+    ``num_timesteps`` is the unrounded count. Returns the hits. The output
+    array (host memory, sized for one hit per merge window) is allocated here,
+    by the caller of the entry point."""
+    L = lib()
+    window = drift_parameters(num_timesteps, foff, tsamp, max_drift)[2]
+    cap = max(1, -(-input.num_channels // max(window, 1)))
+    out = (ScHit * cap)()
+    n = ctypes.c_size_t(0)
+    check(L.sc_dd_search_static(int(num_timesteps), input.num_timesteps, input.num_channels,
+                                foff, tsamp, input.data.data_ptr(), input.dtype,
+                                input.row_stride, ctypes.byref(metadata._sc_meta()), beam,
+                                coarse_channel, max_drift, min_drift, snr_threshold, out, cap,
+                                ctypes.byref(n), _stream_ptr(stream)))
+    if n.value > cap:
+        raise SeticoreError(SC_ERR_INTERNAL, "more hits than merge windows")
+    return list(out[:n.value])
 
 
 # ---------------------------------------------------------------------------

@@ -31,6 +31,7 @@ DEVICE_STRICT = [f for f in DEVICE if f != "-fno-honor-nans"]
 
 SOURCES = [
     ("dd_kernels.hip", DEVICE),
+    ("dd_static.hip", DEVICE_STRICT),
     ("pn_kernels.hip", DEVICE),
     ("dd_capi.cpp", []),
     ("pn_capi.cpp", []),

@@ -302,9 +302,11 @@ void host_exact_stats(const float* colsums, int n, float* median_out, float* std
   *std_out = std_dev;
 }
 
+}  // namespace
+
 // DedopplerHit constructor (dedoppler_hit.cpp:11-23).
-sc_hit make_hit(const sc_meta* meta, int index, int drift_steps, double drift_rate, float snr,
-                int beam, int coarse_channel, int num_timesteps, float power) {
+sc_hit sc::make_hit(const sc_meta* meta, int index, int drift_steps, double drift_rate,
+                    float snr, int beam, int coarse_channel, int num_timesteps, float power) {
   sc_hit h;
   h.index = index;
   h.drift_steps = drift_steps;
@@ -320,6 +322,8 @@ sc_hit make_hit(const sc_meta* meta, int index, int drift_steps, double drift_ra
   h.frequency = meta->fch1 + global_index * meta->foff;
   return h;
 }
+
+namespace {
 
 // dedoppler.cu:264-319 on host arrays.
 void host_merge(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
@@ -354,8 +358,8 @@ void host_merge(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int beam,
     double drift_rate = drift_bins * dd->resolution;
     float snr = (cand_sum - median) / std_dev;
     if (std::fabs(drift_rate) >= min_drift) {
-      dd->hits.push_back(make_hit(meta, cand, drift_bins, drift_rate, snr, beam, coarse_channel,
-                                  dd->num_timesteps, cand_sum));
+      dd->hits.push_back(sc::make_hit(meta, cand, drift_bins, drift_rate, snr, beam,
+                                      coarse_channel, dd->num_timesteps, cand_sum));
     }
   }
 }
@@ -708,8 +712,8 @@ sc_status device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int b
     double drift_rate = drift_bins * dd->resolution;
     float snr = (cand_sum - median) / std_dev;
     if (std::fabs(drift_rate) >= min_drift)
-      out->push_back(make_hit(meta, c.x, drift_bins, drift_rate, snr, beam, coarse_channel,
-                              num_timesteps, cand_sum));
+      out->push_back(sc::make_hit(meta, c.x, drift_bins, drift_rate, snr, beam,
+                                  coarse_channel, num_timesteps, cand_sum));
   }
   return SC_OK;
 }

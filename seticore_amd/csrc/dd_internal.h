@@ -6,6 +6,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "../../include/seticore_amd.h"
+
 namespace sc {
 
 // Drift-block range and merge window of a search (dedoppler.cu:195-198, 274):
@@ -24,6 +26,12 @@ inline DriftRange drift_range_of(double resolution, int drift_timesteps, double 
   r.window = (int)(2 * ceil(r.nmd * drift_timesteps));
   return r;
 }
+
+// A hit as the reference's DedopplerHit constructor forms it
+// (dedoppler_hit.cpp:11-23): frequency from the global channel index, beam -1
+// unless it is a coherent beam of the metadata.
+sc_hit make_hit(const sc_meta* meta, int index, int drift_steps, double drift_rate, float snr,
+                int beam, int coarse_channel, int num_timesteps, float power);
 
 // Default one-pass kernel shape for T_r (channels per lane, drift blocks per
 // LDS window or workgroup, window mode / kernel selector), as measured best.
