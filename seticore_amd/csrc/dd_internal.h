@@ -60,6 +60,11 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
                                  const int* tile_list = nullptr,
                                  const int* tile_count = nullptr, bool block_keys = false);
 int fused_tile_width(int T, int C);
+// The T_r <= 16 kernel of launch_fused_tree_top (dd_fused16.hip), same arguments.
+hipError_t launch_fused16(const void* x, int dtype, long stride, int T, int N, int bmin,
+                          int nblocks, int C, int G, int P, top_key_t* top_key, float* colsum,
+                          hipStream_t stream, const int* tile_list, const int* tile_count,
+                          bool block_keys);
 
 // Whether launch_fused_tree_top has a kernel for (T, C, G, P). At T = 32 the
 // `prefetch` value kStaged32 selects the LDS-staged kernel of T = 64..256
