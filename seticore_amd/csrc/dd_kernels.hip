@@ -668,8 +668,40 @@ __device__ __forceinline__ void staged_load(const In* __restrict__ x, long strid
   }
 }
 
+// R_{q0+qq}[k][s] for s = 4*lane + j - k*qq into LDS row qq*M + k. With
+// d = k*qq = 4m + e, element j goes to column j - e of position lane - m, or
+// to column j - e + 4 of position lane - m - 1; e = ((k mod 4) * qq) mod 4 is a
+// compile-time constant here (K4 = k mod 4), so every store is a fixed offset
+// from one lane address and only the two position bounds are tested.
+template <int M, int G_IDX, int K4>
+__device__ __forceinline__ void staged_store(const float (&v)[16][Staged<M>::C], int k,
+                                             lds_float* R) {
+  using St = Staged<M>;
+  constexpr int C = St::C;
+  constexpr int Q0 = G_IDX * St::QG;
+  const int lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int qq = 0; qq < St::QG; ++qq) {
+    const int e = (K4 * qq) & 3;
+    const int m = (k * qq) >> 2;
+    const int pos = lane - m;
+    lds_float* row = R + (qq * M + k) * St::kRow + pos;
+    const bool in0 = pos >= 0 && pos < St::kSub;
+    const bool in1 = pos >= 1 && pos <= St::kSub;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int jj = j - e;
+      if (jj >= 0) {
+        if (in0) row[jj * St::kSub] = v[Q0 + qq][j];
+      } else {
+        if (in1) row[(jj + C) * St::kSub - 1] = v[Q0 + qq][j];
+      }
+    }
+  }
+}
+
 // Phase 1 for time block k: the partial tree of the loaded rows, and
-// R_{q0+qq}[k][s] for s = 4*lane + j - k*qq into LDS row qq*M + k.
+// R_{q0+qq}[k] for the group's offsets into LDS.
 template <int M, int G_IDX>
 __device__ __forceinline__ void staged_phase1(float (&v)[16][Staged<M>::C], int k,
                                               lds_float* R) {
@@ -679,22 +711,11 @@ __device__ __forceinline__ void staged_phase1(float (&v)[16][Staged<M>::C], int 
   const int lane = threadIdx.x % kWave;
   tree_range<C, Q0, Q0 + St::QG - 1>(v);
   if (lane < St::kLevel16Lanes) {
-#pragma unroll
-    for (int qq = 0; qq < St::QG; ++qq) {
-      // s = 4*(lane - m) + (j - e) with d = k*qq = 4m + e: column j - e of
-      // position lane - m, or column j - e + 4 of position lane - m - 1.
-      const int d = k * qq;
-      const int m = d >> 2;
-      const int e = d & 3;
-      lds_float* row = R + (qq * M + k) * St::kRow;
-      const int pos = lane - m;
-#pragma unroll
-      for (int j = 0; j < C; ++j) {
-        const int jj = j - e;
-        const int col = jj >= 0 ? jj : jj + C;
-        const int p = jj >= 0 ? pos : pos - 1;
-        if (p >= 0 && p < St::kSub) row[col * St::kSub + p] = v[Q0 + qq][j];
-      }
+    switch (k & 3) {
+      case 0: staged_store<M, G_IDX, 0>(v, k, R); break;
+      case 1: staged_store<M, G_IDX, 1>(v, k, R); break;
+      case 2: staged_store<M, G_IDX, 2>(v, k, R); break;
+      default: staged_store<M, G_IDX, 3>(v, k, R); break;
     }
   }
 }
