@@ -184,6 +184,12 @@ def main():
                     help="plain: the file-scale search (default); half: the same with "
                          "FP16/BF16 storage; batched: uneven beams x coarse channels in one "
                          "graph launch; learned: proposal-gated search")
+    ap.add_argument("--storage", default="f32", choices=["f32", "bf16"],
+                    help="batched workload: FP32 input in waves, or BF16 (normalised) input "
+                         "searched in one launch")
+    ap.add_argument("--wave-gib", type=int, default=0,
+                    help="batched workload: GiB of input per wave (default 96 for FP32, "
+                         "everything for BF16)")
     args = ap.parse_args()
     T, MAX_DRIFT = args.time_steps, args.max_drift
     if T not in (2, 4, 8, 16, 32, 64, 128, 256, 512, 1024):
@@ -482,94 +488,128 @@ def run_half(args, torch, sc):
 def run_batched(args, torch, sc):
     """64 beams x 32 coarse channels: time steps per beam from {16, 32, 64, 256},
     fine channels per coarse channel from {2^17, 2^18, 2^20}, 20 tones per beam
-    at +-5 Hz/s and SNR 10-50, max drift 5 Hz/s. Each beam's 32 coarse
-    channels are generated on the GPU and searched in one graph launch
-    (autotuned shapes); the same data is searched instance by instance with
-    the plain API for comparison (hits must agree)."""
+    at +-5 Hz/s and SNR 10-50, max drift 5 Hz/s (66 drift blocks at every time
+    step count), autotuned kernel shapes. The full input is ~353 GB as float32,
+    more than one GPU holds, so the beams are searched in waves: each wave is
+    as many consecutive beams (mixed time steps) as fit in WAVE_BYTES, generated
+    on the GPU (untimed) and searched by one batch graph launch over all its
+    (beam, coarse channel) instances. With --storage bf16 every instance is
+    normalised to BF16 (FP64 moments on the GPU) and the whole input (~177 GB)
+    is searched by a single graph launch. The same data is searched instance
+    by instance with Dedopplerer::search (the per-instance plain loop, as
+    run_dedoppler.cpp:43-52 and beamforming_pipeline.cpp:230-255 call it);
+    hits must agree instance by instance."""
     from seticore_amd import batch as B
     rng = np.random.default_rng(SEED + 3)
     beams, ccs = 64, 32
     if args.coarse_channels != NUM_CC:
         beams = max(1, args.coarse_channels)
+    bf16 = args.storage == "bf16"
     beam_T = [int(rng.choice([16, 32, 64, 256])) for _ in range(beams)]
     cc_N = [int(rng.choice([1 << 17, 1 << 18, 1 << 20])) for _ in range(ccs)]
     max_drift = 5.0
-    tuning = []
-    t_tune = time.perf_counter()
-    for Tb in sorted(set(beam_T)):
-        for Nc in sorted(set(cc_N)):
-            tuning.append(B.autotune(Tb, Nc, FOFF, TSAMP, max_drift, reps=2))
-    t_tune = time.perf_counter() - t_tune
-    bufs = {Tb: [torch.empty((Tb, Nc), dtype=torch.float32, device="cuda") for Nc in cc_N]
-            for Tb in sorted(set(beam_T))}
-    batches = {}
-    for Tb, xs in bufs.items():
-        inst = [B.instance(x, Tb, FOFF, TSAMP, max_drift, 0.0, SNR_THRESHOLD, beam=-1,
-                           coarse_channel=c) for c, x in enumerate(xs)]
-        batches[Tb] = B.Batch(inst, tuning)
-    plain = {}
-    metas = {Nc: sc.FilterbankMetadata(fch1=8419.0, foff=FOFF, coarse_channel_size=Nc)
-             for Nc in set(cc_N)}
-    t_batch = t_plain = 0.0
-    total_ps = 0
-    mismatched = 0
-    total_hits = 0
+    wave_bytes = (args.wave_gib << 30) if args.wave_gib else (1 << 62 if bf16 else 96 << 30)
+    bytes_per = 2 if bf16 else 4
+    waves, cur, cur_bytes = [], [], 0
     for beam in range(beams):
-        Tb = beam_T[beam]
-        xs = bufs[Tb]
+        nb = beam_T[beam] * sum(cc_N) * bytes_per
+        if cur and cur_bytes + nb > wave_bytes:
+            waves.append(cur)
+            cur, cur_bytes = [], 0
+        cur.append(beam)
+        cur_bytes += nb
+    waves.append(cur)
+    # Tones per beam, drawn before any timing so every run sees the same data.
+    beam_tones = []
+    for beam in range(beams):
         tones = {}
         for _ in range(20):
             c = int(rng.integers(0, ccs))
             drift = float(rng.uniform(-5.0, 5.0))
             tones.setdefault(c, []).append((float(rng.uniform(0.06, 0.94)), drift,
                                             float(rng.uniform(10.0, 50.0))))
-        for c, x in enumerate(xs):
-            Nc = cc_N[c]
-            tl = [(f * Nc, d * TSAMP / (FOFF * 1e6), snr / math.sqrt(Tb))
-                  for f, d, snr in tones.get(c, [])]
-            sc.gen_waterfall(x, Tb, seed=SEED * 7 + beam * 1000 + c, tones=tl)
-            nb = sc.drift_parameters(Tb, FOFF, TSAMP, max_drift)
-            total_ps += (nb[1] - nb[0] + 1) * Tb * Nc
+        beam_tones.append(tones)
+    t_tune = time.perf_counter()
+    tuning = []
+    for Tb in sorted(set(beam_T)):
+        for Nc in sorted(set(cc_N)):
+            tuning.append(B.autotune(Tb, Nc, FOFF, TSAMP, max_drift, reps=2))
+    t_tune = time.perf_counter() - t_tune
+    metas = {Nc: sc.FilterbankMetadata(fch1=8419.0, foff=FOFF, coarse_channel_size=Nc)
+             for Nc in set(cc_N)}
+    dtype = sc.SC_BF16 if bf16 else sc.SC_F32
+    plain = {}
+    t_batch = t_plain = t_gen = 0.0
+    total_ps = mismatched = total_hits = 0
+    graph_nodes = 0
+    for wave in waves:
+        t0 = time.perf_counter()
+        bufs, inst = [], []
+        for beam in wave:
+            Tb = beam_T[beam]
+            for c, Nc in enumerate(cc_N):
+                x = torch.empty((Tb, Nc), dtype=torch.float32, device="cuda")
+                tl = [(f * Nc, d * TSAMP / (FOFF * 1e6), snr / math.sqrt(Tb))
+                      for f, d, snr in beam_tones[beam].get(c, [])]
+                sc.gen_waterfall(x, Tb, seed=SEED * 7 + beam * 1000 + c, tones=tl)
+                if bf16:
+                    x = sc.normalize(x, sc.SC_BF16)[0]
+                bufs.append((beam, c, Tb, Nc, x))
+                inst.append(B.instance(x, Tb, FOFF, TSAMP, max_drift, 0.0, SNR_THRESHOLD,
+                                       beam=-1, coarse_channel=c, dtype=dtype,
+                                       metadata=metas[Nc]))
+                nb = sc.drift_parameters(Tb, FOFF, TSAMP, max_drift)
+                total_ps += (nb[1] - nb[0] + 1) * Tb * Nc
+        torch.cuda.synchronize()
+        batch = B.Batch(inst, tuning)  # plan and capture: once per wave, untimed
+        graph_nodes += batch.info()["graph_nodes"]
+        t_gen += time.perf_counter() - t0
+        batch.run()  # warm-up launch
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        batches[Tb].run()
+        batch.run()
         t_batch += time.perf_counter() - t0
+        batch_hits = [batch.hits(i) for i in range(len(inst))]
         # Per-instance plain loop over the same data.
         t0 = time.perf_counter()
         loop_hits = []
-        for c, x in enumerate(xs):
-            Nc = cc_N[c]
+        for beam, c, Tb, Nc, x in bufs:
             key = (Tb, Nc)
             if key not in plain:
                 dd = sc.Dedopplerer(Tb, Nc, FOFF, TSAMP)
                 dd.set_stats_mode(sc.SC_STATS_DEVICE)
+                dd.set_graphs(False)
                 plain[key] = dd
             out = []
-            plain[key].search(sc.FilterbankBuffer(Tb, Nc, data=x), metas[Nc], sc.NO_BEAM, c,
-                              max_drift, 0.0, SNR_THRESHOLD, out)
+            plain[key].search(sc.FilterbankBuffer(Tb, Nc, data=x, dtype=dtype), metas[Nc],
+                              sc.NO_BEAM, c, max_drift, 0.0, SNR_THRESHOLD, out)
             loop_hits.append(out)
         torch.cuda.synchronize()
         t_plain += time.perf_counter() - t0
-        for c in range(ccs):
-            a = [h.key() + (h.snr,) for h in batches[Tb].hits(c)]
-            b = [h.key() + (h.snr,) for h in loop_hits[c]]
+        for a, b in zip(batch_hits, loop_hits):
             total_hits += len(b)
-            mismatched += a != b
+            mismatched += [h.key() + (h.snr,) for h in a] != [h.key() + (h.snr,) for h in b]
+        del batch, bufs, inst, batch_hits, loop_hits
+        torch.cuda.empty_cache()
     value = total_ps / t_batch / 1e9
-    _line(value, "Gpath-sums/s", 1000 * t_batch, args, "f32",
+    _line(value, "Gpath-sums/s", 1000 * t_batch, args,
+          "bf16 storage, f32 accumulate" if bf16 else "f32",
           "synthetic (seeded exponential noise + 20 tones per beam, generated on the GPU "
-          "beam by beam; generation untimed)",
+          "wave by wave; generation, normalisation and graph capture untimed)",
           {"workload": "batched uneven search: %d beams x %d coarse channels, time steps per "
                        "beam from {16,32,64,256}, channels per coarse channel from "
-                       "{2^17,2^18,2^20}, max drift 5 Hz/s, SNR 10, one graph launch per beam"
-                       % (beams, ccs),
+                       "{2^17,2^18,2^20}, max drift 5 Hz/s, SNR 10, %s storage; %d wave(s) "
+                       "of mixed-time-step beams, one graph launch per wave"
+                       % (beams, ccs, "BF16" if bf16 else "FP32", len(waves)),
            "path_sums": total_ps, "beam_time_steps": beam_T, "coarse_channel_widths": cc_N,
+           "waves": [len(w) for w in waves], "graph_nodes": graph_nodes,
            "plain_loop_gpath_per_s": round(total_ps / t_plain / 1e9, 3),
            "speedup_vs_plain_loop": round(t_plain / t_batch, 3),
            "instances_with_different_hits": mismatched, "hits": total_hits,
-           "autotune_seconds": round(t_tune, 2),
+           "autotune_seconds": round(t_tune, 2), "setup_seconds": round(t_gen, 2),
            "tuning": [t.as_dict() for t in tuning]},
-          steps_note="one pass over all beams (the steps/warmup flags do not apply)")
+          steps_note="one timed launch per wave after one warm-up launch (the steps/warmup "
+                     "flags do not apply)")
 
 
 def run_learned(args, torch, sc):
