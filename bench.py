@@ -184,6 +184,9 @@ def main():
                     help="plain: the file-scale search (default); half: the same with "
                          "FP16/BF16 storage; batched: uneven beams x coarse channels in one "
                          "graph launch; learned: proposal-gated search")
+    ap.add_argument("--snr-threshold", type=float, default=SNR_THRESHOLD,
+                    help="learned workload: SNR threshold of the plain search that labels the "
+                         "blocks and of the searches timed (default 10)")
     ap.add_argument("--storage", default="f32", choices=["f32", "bf16"],
                     help="batched workload: FP32 input in waves, or BF16 (normalised) input "
                          "searched in one launch")
@@ -621,15 +624,24 @@ def run_learned(args, torch, sc):
     timed against the plain search on 64 further coarse channels (FP32 and
     BF16 storage), with hit recall versus the plain search."""
     from seticore_amd import proposal as P
+    snr = args.snr_threshold
     t0 = time.perf_counter()
-    train = P.make_dataset(256, N, seed=11)
-    val = P.make_dataset(32, N, seed=12)
+    train = P.make_dataset(256, N, seed=11, snr_threshold=snr)
+    val = P.make_dataset(32, N, seed=12, snr_threshold=snr)
     net = sc.ProposalNet(max_batch=256, seed=13)
-    hist = P.train(net, train, epochs=30, batch=256, lr=3e-3)
+    hist = P.train(net, train)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     z = P.scores(net, val.tiles)
-    thresholds = {r: P.threshold_for_recall(z, val.labels, r) for r in (0.9, 0.99, 0.995, 1.0)}
+    zh = P.heuristic_scores(val.tiles)
+    recalls = (0.9, 0.99, 0.995, 1.0)
+    thresholds = {r: P.threshold_for_recall(z, val.labels, r) for r in recalls}
+    # Held-out block fractions each method flags at the same block recall.
+    val_flagged = {}
+    for r in recalls:
+        th, hh = thresholds[r], P.threshold_for_recall(zh, val.labels, r)
+        val_flagged[str(r)] = {"network": round(float((z > th).float().mean()), 4),
+                               "brightest_pixel": round(float((zh > hh).float().mean()), 4)}
     del train, val
     num_cc = args.coarse_channels
     rng = np.random.default_rng(SEED + 5)
@@ -651,7 +663,7 @@ def run_learned(args, torch, sc):
 
     def plain(cc):
         out = []
-        dd.search(b32[cc], meta, sc.NO_BEAM, cc, MAX_DRIFT, 0.0, SNR_THRESHOLD, out)
+        dd.search(b32[cc], meta, sc.NO_BEAM, cc, MAX_DRIFT, 0.0, snr, out)
         plain_hits[cc] = {(h.index, h.drift_steps) for h in out}
 
     t_plain = _timed_searches(torch, plain, num_cc, args.steps, args.warmup)
@@ -665,7 +677,7 @@ def run_learned(args, torch, sc):
             def gated(cc):
                 out = []
                 flagged[0] += dd.search_proposed(bufs[cc], net, meta, sc.NO_BEAM, cc, MAX_DRIFT,
-                                                 0.0, SNR_THRESHOLD, thr, out)
+                                                 0.0, snr, thr, out)
                 if name == "f32":
                     found[0] += len(plain_hits[cc] & {(h.index, h.drift_steps) for h in out})
 
@@ -683,9 +695,12 @@ def run_learned(args, torch, sc):
           1000 * t_plain / main_pt["speedup_vs_plain"], args, "bf16 network, f32 search",
           "synthetic (learned-proposal generator: tones SNR 8-50, +-10 Hz/s, RFI, bursts)",
           {"workload": "learned proposal + full fallback: %d coarse channels x 2^20 x 16, "
-                       "4096-channel blocks, thresholds fixed on held-out data" % num_cc,
+                       "4096-channel blocks, SNR threshold %g, thresholds fixed on held-out "
+                       "data" % (num_cc, snr),
+           "snr_threshold": snr,
            "plain_gpath_per_s": round(per_cc * num_cc / t_plain / 1e9, 3),
            "plain_hits": nhits, "points": points,
+           "held_out_blocks_flagged_at_block_recall": val_flagged,
            "train_seconds_incl_data": round(t_train, 2),
            "final_train_loss": round(hist[-1], 4)})
 

@@ -81,10 +81,10 @@ class Dataset:
         self.recipes = recipes
 
 
-def make_dataset(num_cc, N, seed, label_snr=0.0, **recipe_args):
+def make_dataset(num_cc, N, seed, label_snr=0.0, snr_threshold=SNR_THRESHOLD, **recipe_args):
     """num_cc synthetic coarse channels of N channels: each is searched with
-    the plain search (device statistics) for its labels (hits with SNR >=
-    label_snr), then pooled."""
+    the plain search (device statistics, SNR threshold `snr_threshold`) for its
+    labels (hits with SNR >= label_snr), then pooled."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     nblk = N // BLOCK_CHANNELS
@@ -99,7 +99,7 @@ def make_dataset(num_cc, N, seed, label_snr=0.0, **recipe_args):
         recipe = coarse_channel_recipe(rng, N, **recipe_args)
         synth(x, seed * 100003 + cc, recipe)
         hits = []
-        dd.search(FilterbankBuffer(T, N, data=x), meta, -1, cc, MAX_DRIFT, 0.0, SNR_THRESHOLD,
+        dd.search(FilterbankBuffer(T, N, data=x), meta, -1, cc, MAX_DRIFT, 0.0, snr_threshold,
                   hits)
         st = dd.state()
         stats = torch.tensor([st["median"], st["std"]], dtype=torch.float32, device="cuda")
@@ -111,8 +111,12 @@ def make_dataset(num_cc, N, seed, label_snr=0.0, **recipe_args):
     return Dataset(tiles, torch.from_numpy(labels).cuda(), all_hits, recipes)
 
 
-def train(net, data, epochs=6, batch=256, lr=2e-3, pos_weight=None, seed=0, log=None):
-    """Adam on shuffled mini-batches; returns the mean loss of each epoch."""
+def train(net, data, epochs=20, batch=256, lr=1e-2, pos_weight=1.0, seed=0, log=None):
+    """Adam on shuffled mini-batches; returns the mean loss of each epoch.
+    Defaults from a sweep on held-out coarse channels (lr 3e-4 .. 1e-2, 20 or
+    80 epochs, unweighted or class-balanced loss): lr 1e-2, 20 epochs and an
+    unweighted loss flag the fewest blocks at 90 % and 99 % block recall.
+    pos_weight=None balances the classes instead."""
     torch = _torch()
     n = data.tiles.shape[0]
     batch = min(batch, net.max_batch, n)

@@ -132,24 +132,30 @@ def test_gated_search_with_every_block_flagged_equals_plain(gpu):
     assert none == []
 
 
-def test_training_gains_over_the_untrained_network(gpu):
-    # What training buys: on held-out coarse channels, at the threshold that
-    # keeps 90 % of the plain search's hit blocks, the trained network flags
-    # fewer blocks than the same network before training; its loss falls.
-    import torch
+def test_trained_network_beats_the_brightest_pixel_heuristic(gpu):
+    # What training buys, against the non-learned baseline: on 80 held-out
+    # coarse channels, at the threshold that keeps 90 % of the plain search's
+    # hit blocks, the trained network flags fewer blocks than the brightest
+    # pooled pixel does (and than the same network untrained). The bar is the
+    # claim itself - strictly fewer blocks than the heuristic - with the
+    # margin printed; measured: trained 0.748, heuristic 0.783, untrained
+    # 0.909 (deterministic: seeded data, fixed-order gradient sums).
     from seticore_amd import proposal as P
     N = 1 << 18
-    train = P.make_dataset(96, N, seed=31)
-    val = P.make_dataset(24, N, seed=32)
+    train = P.make_dataset(640, N, seed=31)
+    val = P.make_dataset(80, N, seed=32)
     net = sc.ProposalNet(max_batch=256, seed=33)
     z0 = P.scores(net, val.tiles)
-    hist = P.train(net, train, epochs=20, batch=256, lr=3e-3)
+    hist = P.train(net, train)
     z1 = P.scores(net, val.tiles)
+    zh = P.heuristic_scores(val.tiles)
     flags = []
-    for z in (z0, z1):
+    for z in (z0, z1, zh):
         thr = P.threshold_for_recall(z, val.labels, 0.9)
         flags.append(float((z > thr).float().mean()))
-    print("loss %.3f -> %.3f; flagged at 90%% block recall: untrained %.3f, trained %.3f" % (
-        hist[0], hist[-1], flags[0], flags[1]))
+    print("loss %.3f -> %.3f; blocks flagged at 90%% block recall: untrained %.3f, trained "
+          "%.3f, brightest pixel %.3f (margin %.3f)" % (hist[0], hist[-1], flags[0], flags[1],
+                                                          flags[2], flags[2] - flags[1]))
     assert hist[-1] < hist[0]
-    assert flags[1] < flags[0] - 0.01  # measured: 0.902 -> 0.875 (deterministic run)
+    assert flags[1] < flags[2]
+    assert flags[1] < flags[0]
