@@ -94,11 +94,17 @@ def cpu_baseline(data_cc, threads):
     return dt, res
 
 
-def load_traffic():
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3
-    counter summary (profiles/), or None."""
-    path = os.path.join(ROOT, "profiles", "fused_tree_cfg2_traffic.json")
-    if not os.path.exists(path):
+# Committed rocprofv3 counter summaries (profiles/): HBM bytes per launch of
+# the dominant kernel, by (time steps, max drift) of the measured workload.
+TRAFFIC_FILES = {(16, 10.0): "fused_tree_cfg2_traffic.json",
+                 (256, 5.0): "staged_t256_traffic.json"}
+
+
+def load_traffic(T_r, max_drift):
+    """HBM bytes per launch of the dominant kernel for this workload, or None."""
+    name = TRAFFIC_FILES.get((T_r, max_drift))
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f).get("hbm_bytes_per_launch")
@@ -293,23 +299,38 @@ def main():
     value = total_ps / (elapsed / args.steps) / 1e9
 
     # Roofline of the dominant kernel (tree + top-path tracking over all drift
-    # blocks, one coarse channel per launch). It is bound by vector ALU issue:
-    # the input is read from HBM about once per launch and the drift blocks
-    # are served from LDS and L2. achieved = algorithmic VALU lane-operations
-    # per launch (path sums x (log2(T_r) adds + 1/2 max)) / launch time, peak
-    # = 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz. Beside it: the HBM traffic
-    # the counters measured per launch (profiles/) as a fraction of 8 TB/s,
-    # and the reference-dataflow-equivalent byte rate (D*T*N*4 + T*N*4 + 12*N
-    # bytes per launch: the reference streams the waterfall once per drift
-    # block, dedoppler.cu:214-227), which is not a fraction of any roof.
+    # blocks, one coarse channel per launch).
+    # * T_r <= 16: bound by vector ALU issue. The input is read from HBM about
+    #   once per launch and the drift blocks are served from LDS and L2.
+    #   achieved = algorithmic VALU lane-operations per launch (path sums x
+    #   (log2(T_r) adds + 1/2 max)) / launch time, peak = 256 CUs x 4 SIMDs x
+    #   16 lanes x 2.4 GHz; beside it the HBM traffic the counters measured
+    #   per launch (profiles/) as a fraction of 8 TB/s, and the
+    #   reference-dataflow-equivalent byte rate, which is not a fraction of a
+    #   roof here.
+    # * T_r >= 32: bound by memory. The shear moves a row by T_r - 1 channels
+    #   per drift block, more than a tile that fits in LDS, so the input is
+    #   streamed once per drift block: the reference's dataflow bytes
+    #   (D*T*N*4 + T*N*4 + 12*N per launch, dedoppler.cu:214-227) are the
+    #   floor. achieved = those bytes / launch time, peak 8 TB/s; the VALU
+    #   fraction is reported beside it.
     avg_ms = kernel_ms / max(1, launches)
     ops_per_ps = alg_valu_ops_per_path_sum(T)
     alg_ops = per_cc * ops_per_ps
     achieved_tops = alg_ops / (avg_ms * 1e-3) / 1e12
     peak_tops = VALU_PEAK_OPS / 1e12
     ref_dataflow_bytes = nblocks * T * N * 4 + T * N * 4 + 12 * N
-    traffic = load_traffic() if (T, MAX_DRIFT) == (16, 10.0) else None
+    traffic = load_traffic(T, MAX_DRIFT)
+    ref_gbs = ref_dataflow_bytes / (avg_ms * 1e-3) / 1e9
 
+    rl_common = {
+        "kernel": "fused_tree_top_kernel" if T <= 16 else "staged_tree_kernel",
+        "kernel_ms_per_launch": round(avg_ms, 4),
+        "path_sums_per_launch": per_cc,
+        "hbm_traffic_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "hbm_traffic_frac": (round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                             if traffic else None),
+    }
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -332,25 +353,30 @@ def main():
             "hits_per_step": len(hits), "step": mode,
             "per_call_loop_gpath_per_s": round(total_ps / (loop_elapsed / args.steps) / 1e9, 3),
         },
-        "roofline": {
+        "roofline": ({
             "bound": "valu",
             "achieved": round(achieved_tops, 3),
             "peak": round(peak_tops, 3),
             "unit": "TOP/s (VALU lane-operations, non-packed)",
             "frac": round(achieved_tops / peak_tops, 4),
             "traffic": traffic,
-            "kernel": "fused_tree_top_kernel" if T <= 16 else "staged_tree_kernel",
-            "kernel_ms_per_launch": round(avg_ms, 4),
-            "path_sums_per_launch": per_cc,
             "alg_ops_per_path_sum": ops_per_ps,
             "alg_ops_per_launch": alg_ops,
-            "hbm_traffic_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
-            "hbm_traffic_frac": (round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                                 if traffic else None),
-            "reference_dataflow_equiv_gbs": round(ref_dataflow_bytes / (avg_ms * 1e-3) / 1e9, 1),
+            "reference_dataflow_equiv_gbs": round(ref_gbs, 1),
             "reference_dataflow_bytes_per_launch": ref_dataflow_bytes,
-        },
+        } if T <= 16 else {
+            "bound": "hbm",
+            "achieved": round(ref_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ref_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "alg_bytes_per_launch": ref_dataflow_bytes,
+            "valu_frac": round(achieved_tops / peak_tops, 4),
+        }),
     }
+
+    line["roofline"].update(rl_common)
 
     if not args.no_cpu_baseline:
         host = host_description()
@@ -380,6 +406,26 @@ def main():
             "host": host,
             "hits_vs_oracle": diff,
         }
+
+    # The same step after the loader's DC-spike fix (h5_reader.cpp:202-209:
+    # the centre bin of every coarse channel replaced by the mean of its two
+    # neighbours), applied on the GPU in place: hits and per-call throughput
+    # with the fix, beside the unconditioned figures above.
+    dc = N // 2
+    before = sum(1 for h in loop_hits if abs(h.index - dc) <= 1)
+    for cc in range(num_cc):
+        sc.condition_coarse_channel(data[cc], T, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fixed_hits = loop_step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    line["config"]["dc_spike_fix"] = {
+        "hits_per_step": len(fixed_hits),
+        "hits_at_the_centre_bin": sum(1 for h in fixed_hits if abs(h.index - dc) <= 1),
+        "hits_at_the_centre_bin_without_fix": before,
+        "per_call_loop_gpath_per_s": round(total_ps / dt / 1e9, 3),
+    }
     print(json.dumps(line), flush=True)
 
 
