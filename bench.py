@@ -668,7 +668,10 @@ def run_learned(args, torch, sc):
     by the plain search, thresholds are fixed on 32 held-out coarse channels
     for block recalls of 0.9, 0.99, 0.995 and 1.0, and the gated search is
     timed against the plain search on 64 further coarse channels (FP32 and
-    BF16 storage), with hit recall versus the plain search."""
+    BF16 storage), with hit recall versus the plain search - per call, and
+    batched (one graph for all coarse channels, the network of one beside the
+    tree of the previous one) against the plain batch. The headline value is
+    the combined form: gated, batched, BF16 storage, at 0.995 block recall."""
     from seticore_amd import proposal as P
     snr = args.snr_threshold
     t0 = time.perf_counter()
@@ -714,6 +717,29 @@ def run_learned(args, torch, sc):
 
     t_plain = _timed_searches(torch, plain, num_cc, args.steps, args.warmup)
     nhits = sum(len(v) for v in plain_hits.values())
+    from seticore_amd import batch as B
+    inst = {name: [B.instance(src[cc], T, FOFF, TSAMP, MAX_DRIFT, 0.0, snr, coarse_channel=cc,
+                              dtype=dt, metadata=meta) for cc in range(num_cc)]
+            for name, src, dt in (("f32", data, sc.SC_F32), ("bf16", half, sc.SC_BF16))}
+
+    def timed_batch(b):
+        for _ in range(args.warmup):
+            b.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps
+
+    t_plain_batch = {}
+    for name in ("f32", "bf16"):
+        pb = B.Batch(inst[name])
+        t_plain_batch[name] = timed_batch(pb)
+        if name == "f32":
+            assert all({(h.index, h.drift_steps) for h in pb.hits(cc)} == plain_hits[cc]
+                       for cc in range(num_cc)), "plain batch differs from the per-call search"
+        del pb
     points = []
     for rec, thr in sorted(thresholds.items()):
         for name, bufs in (("f32", b32), ("bf16", b16)):
@@ -731,20 +757,44 @@ def run_learned(args, torch, sc):
             frac = flagged[0] / (num_cc * (N // 4096))
             recall = found[0] / max(nhits, 1)
             sec = _timed_searches(torch, gated, num_cc, args.steps, 0)
+            gb = B.Batch(inst[name], proposal=net, logit_threshold=thr)
+            sec_b = timed_batch(gb)
+            bflag = sum(gb.flagged(cc)[1] for cc in range(num_cc))
+            assert bflag == round(frac * num_cc * (N // 4096)), \
+                "gated batch flags differ from the per-call search"
+            brec = (sum(len(plain_hits[cc] & {(h.index, h.drift_steps) for h in gb.hits(cc)})
+                        for cc in range(num_cc)) / max(nhits, 1)) if name == "f32" else None
+            del gb
             points.append({"val_block_recall": rec, "storage": name,
                            "threshold": round(thr, 4), "flagged_fraction": round(frac, 4),
                            "hit_recall_vs_plain": round(recall, 4) if name == "f32" else None,
                            "equiv_gpath_per_s": round(per_cc * num_cc / sec / 1e9, 3),
-                           "speedup_vs_plain": round(t_plain / sec, 3)})
-    main_pt = [p for p in points if p["val_block_recall"] == 0.995 and p["storage"] == "f32"][0]
-    _line(main_pt["equiv_gpath_per_s"], "Gpath-sums/s (equivalent: full-search path sums / time)",
-          1000 * t_plain / main_pt["speedup_vs_plain"], args, "bf16 network, f32 search",
+                           "speedup_vs_plain": round(t_plain / sec, 3),
+                           "batched": {
+                               "equiv_gpath_per_s": round(per_cc * num_cc / sec_b / 1e9, 3),
+                               "speedup_vs_plain_per_call_f32": round(t_plain / sec_b, 3),
+                               "speedup_vs_plain_batch_f32": round(t_plain_batch["f32"] / sec_b,
+                                                                   3),
+                               "speedup_vs_plain_batch_same_storage":
+                                   round(t_plain_batch[name] / sec_b, 3),
+                               "hit_recall_vs_plain": (round(brec, 4) if brec is not None
+                                                       else None)}})
+    main_pt = [p for p in points if p["val_block_recall"] == 0.995 and p["storage"] == "bf16"][0]
+    main_b = main_pt["batched"]
+    _line(main_b["equiv_gpath_per_s"], "Gpath-sums/s (equivalent: full-search path sums / time)",
+          1000 * t_plain / main_b["speedup_vs_plain_per_call_f32"], args,
+          "bf16 storage and network, f32 tree accumulation",
           "synthetic (learned-proposal generator: tones SNR 8-50, +-10 Hz/s, RFI, bursts)",
-          {"workload": "learned proposal + full fallback: %d coarse channels x 2^20 x 16, "
-                       "4096-channel blocks, SNR threshold %g, thresholds fixed on held-out "
-                       "data" % (num_cc, snr),
+          {"workload": "learned proposal + batched + BF16 storage: %d coarse channels x 2^20 x "
+                       "16, 4096-channel blocks, SNR threshold %g, logit threshold for 0.995 "
+                       "block recall fixed on held-out data; against the plain per-call FP32 "
+                       "search" % (num_cc, snr),
            "snr_threshold": snr,
+           "combined_speedup_vs_plain_per_call_f32": main_b["speedup_vs_plain_per_call_f32"],
+           "combined_speedup_vs_plain_batch_f32": main_b["speedup_vs_plain_batch_f32"],
            "plain_gpath_per_s": round(per_cc * num_cc / t_plain / 1e9, 3),
+           "plain_batch_gpath_per_s": {k: round(per_cc * num_cc / v / 1e9, 3)
+                                       for k, v in t_plain_batch.items()},
            "plain_hits": nhits, "points": points,
            "held_out_blocks_flagged_at_block_recall": val_flagged,
            "train_seconds_incl_data": round(t_train, 2),

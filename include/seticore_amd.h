@@ -35,7 +35,7 @@ typedef enum {
 
 /* Storage type of the power samples handed to sc_dd_search. The reference
  * only has float32 (filterbank_buffer.h:35). The half-width forms hold
- * normalised power (see sc_dd_normalize) and are accumulated in float32. */
+ * normalised power (see sc_normalize) and are accumulated in float32. */
 typedef enum { SC_F32 = 0, SC_F16 = 1, SC_BF16 = 2 } sc_dtype;
 
 /* Where the column-sum statistics and the window merge run.
@@ -208,13 +208,19 @@ sc_status sc_dd_taylor_tree(sc_dd* dd, const void* data, int dtype,
                             int rounded_timesteps, int num_channels, long row_stride,
                             int drift_block, float* out, void* stream);
 
-/* Per-coarse-channel normalisation for the half-width storage forms: computes
- * the FP64 mean and standard deviation of all T*N samples on the GPU and
- * writes (x - mean) / std, clamped to +-clamp, as `out_dtype` ([T][N]).
- * No counterpart in the reference (it only stores float32). */
+/* Per-coarse-channel normalisation for the half-width storage forms: writes
+ * (x - loc) / std, clamped to +-clamp, as `out_dtype` ([T][N]), where loc is
+ * the median of the T-sample column sums over T and std the FP64 standard
+ * deviation of all T*N samples, both computed on the GPU. With loc at the
+ * median (not the mean, which narrowband RFI can pull far above the noise)
+ * the normalised column sums have median ~0, so a positive SNR threshold is a
+ * positive path sum and the search's zero start for top path sums
+ * (dedoppler.cu:205) never forms candidates. loc_out / std_out (may be NULL)
+ * receive loc and std. No counterpart in the reference (it only stores
+ * float32). */
 sc_status sc_normalize(const float* data, int rounded_timesteps, int num_channels,
-                       long row_stride, int out_dtype, float clamp, void* out,
-                       double* mean_out, double* std_out, void* stream);
+                       long row_stride, int out_dtype, float clamp, void* out, double* loc_out,
+                       double* std_out, void* stream);
 
 /* Synthetic waterfall generator used by the tests and the benchmark (not part
  * of the reference): exponential(1) noise from a counter-based hash of
@@ -406,6 +412,19 @@ sc_status sc_batch_run(sc_batch* b, void* stream);
 /* Hits of one instance from the last run (same contract as sc_dd_copy_hits). */
 sc_status sc_batch_copy_hits(const sc_batch* b, int instance, sc_hit* hits, size_t capacity,
                              size_t* num_hits);
+/* Gated batch: every instance searched as sc_dd_search_proposed does (16 time
+ * steps, a multiple of 4096 channels, at most pn's max_batch blocks each).
+ * Each run is direct launches on three streams (not a graph): the trees on
+ * one, each instance's column sums, statistics, network and tile list on
+ * another (beside the previous instance's tree), its merge on the third. pn must
+ * outlive the batch; its weights are read at every run. sc_batch_info reports
+ * 0 graph nodes for it. */
+sc_status sc_batch_create_proposed(const sc_instance* instances, int count,
+                                   const sc_tuning* tuning, int num_tuning, sc_pn* pn,
+                                   float logit_threshold, sc_batch** out);
+/* Gated batch, last run: channel tiles searched and blocks flagged for one instance. */
+sc_status sc_batch_flagged_blocks(const sc_batch* b, int instance, int* tiles_searched,
+                                  int* flagged_blocks);
 /* Number of distinct shapes and of graph nodes. */
 sc_status sc_batch_info(const sc_batch* b, int* num_shapes, int* graph_nodes);
 sc_status sc_batch_destroy(sc_batch* b);

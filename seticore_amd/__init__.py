@@ -580,9 +580,11 @@ def gen_waterfall(out, num_timesteps, seed, gain=1.0, taper_frac=0.0, dc_spike=0
 
 
 def normalize(data, out_dtype, clamp=65504.0, stream=None, out=None):
-    """Per-coarse-channel normalisation (x - mean) / std with FP64 moments on the
-    GPU, stored as `out_dtype` (into `out` if given, a contiguous [T][N] CUDA
-    tensor of that type). Returns (tensor, mean, std)."""
+    """Per-coarse-channel normalisation (x - loc) / std on the GPU, stored as
+    `out_dtype` (into `out` if given, a contiguous [T][N] CUDA tensor of that
+    type): loc = median column sum / T (so the normalised column sums have
+    median ~0 and a positive SNR threshold is a positive path sum), std = the
+    FP64 deviation of all samples. Returns (tensor, loc, std)."""
     torch = _torch()
     T, N = data.shape
     tdtype = {SC_F32: torch.float32, SC_F16: torch.float16, SC_BF16: torch.bfloat16}[out_dtype]
@@ -591,13 +593,13 @@ def normalize(data, out_dtype, clamp=65504.0, stream=None, out=None):
     elif out.dtype != tdtype or tuple(out.shape) != (T, N) or not out.is_contiguous():
         raise SeticoreError(SC_ERR_INVALID_ARGUMENT, "normalize: out must be a contiguous "
                                                      "[T][N] tensor of the output type")
-    mean = ctypes.c_double()
+    loc = ctypes.c_double()
     sd = ctypes.c_double()
     check(lib().sc_normalize(ctypes.c_void_p(data.data_ptr()), int(T), int(N),
                              int(data.stride(0)), int(out_dtype), float(clamp),
-                             ctypes.c_void_p(out.data_ptr()), ctypes.byref(mean),
+                             ctypes.c_void_p(out.data_ptr()), ctypes.byref(loc),
                              ctypes.byref(sd), _stream_ptr(stream)))
-    return out, mean.value, sd.value
+    return out, loc.value, sd.value
 
 
 # ---------------------------------------------------------------------------

@@ -47,6 +47,11 @@ class ScTuning(ctypes.Structure):
 _SIGS = {
     "sc_batch_create": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                         ctypes.POINTER(ctypes.c_void_p)],
+    "sc_batch_create_proposed": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_float,
+                                 ctypes.POINTER(ctypes.c_void_p)],
+    "sc_batch_flagged_blocks": [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int)],
     "sc_batch_run": [ctypes.c_void_p, ctypes.c_void_p],
     "sc_batch_copy_hits": [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ScHit), ctypes.c_size_t,
                            ctypes.POINTER(ctypes.c_size_t)],
@@ -118,19 +123,28 @@ def multibeam_instances(multibeam, num_timesteps, coarse_channel_size, foff, tsa
 
 
 class Batch:
-    """sc_batch: plan once (graph capture), run many times."""
+    """sc_batch: plan once (graph capture), run many times.
 
-    def __init__(self, instances, tuning=()):
+    With `proposal` (a ProposalNet), every instance is searched as
+    Dedopplerer.search_proposed does, gated at `logit_threshold`; the network
+    of one instance runs beside the tree of the previous one."""
+
+    def __init__(self, instances, tuning=(), proposal=None, logit_threshold=0.0):
         L = _lib()
-        # The graph reads the instances' memory through raw pointers: keep the
-        # source tensors alive as long as the batch.
+        # The graph reads the instances' memory (and the network's) through
+        # raw pointers: keep them alive as long as the batch.
         self._sources = [getattr(i, "_source", None) for i in instances]
+        self._proposal = proposal
         self._inst = (ScInstance * len(instances))(*instances)
         self._tuning = (ScTuning * max(1, len(tuning)))(*tuning)
         h = ctypes.c_void_p()
-        check(L.sc_batch_create(ctypes.cast(self._inst, ctypes.c_void_p), len(instances),
-                                ctypes.cast(self._tuning, ctypes.c_void_p), len(tuning),
-                                ctypes.byref(h)))
+        args = (ctypes.cast(self._inst, ctypes.c_void_p), len(instances),
+                ctypes.cast(self._tuning, ctypes.c_void_p), len(tuning))
+        if proposal is None:
+            check(L.sc_batch_create(*args, ctypes.byref(h)))
+        else:
+            check(L.sc_batch_create_proposed(*args, proposal._h, float(logit_threshold),
+                                             ctypes.byref(h)))
         self._h = h
         self.count = len(instances)
 
@@ -153,6 +167,12 @@ class Batch:
         buf = (ScHit * max(1, n.value))()
         check(L.sc_batch_copy_hits(self._h, i, buf, n.value, ctypes.byref(n)))
         return list(buf)[:n.value]
+
+    def flagged(self, i):
+        """Gated batch, last run: (channel tiles searched, blocks flagged) of instance i."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(_lib().sc_batch_flagged_blocks(self._h, i, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def info(self):
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -610,7 +610,7 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
 sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_threshold,
                               const void* data, int dtype, long row_stride, hipStream_t stream,
                               int4* h_cand = nullptr, float* h_stats = nullptr,
-                              bool stats_done = false) {
+                              bool stats_done = false, bool gated = false) {
   if (!h_cand) h_cand = dd->h_cand;
   if (!h_stats) h_stats = dd->h_stats;
   const int T = dd->rounded, N = dd->num_channels;
@@ -621,7 +621,7 @@ sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_thresho
   const bool bk = block_key_search(dd);
   SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
                                  dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off, dd->cand,
-                                 bk, stream),
+                                 bk, stream, gated),
          "window merge");
   const int nwin = (int)(((long)N + r.window - 1) / r.window);
   if (bk)
@@ -703,6 +703,12 @@ sc_status device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int b
   for (int i = 0; i < nwin; ++i) {
     const int4 c = h_cand[i];
     if (c.x < 0) continue;
+    if (c.w < 0 && c.y == 0)
+      return fail(SC_ERR_UNSUPPORTED,
+                  "the threshold is below zero and channel " + std::to_string(c.x) +
+                      " has no positive path: its hit takes the path offset of an earlier "
+                      "search (dedoppler.cu:305-306), which block-key searches do not keep; "
+                      "use SC_STATS_HOST_EXACT");
     if (c.w < 0)
       return fail(SC_ERR_INTERNAL, "no path offset reproduces the top sum at channel " +
                                        std::to_string(c.x));
@@ -715,6 +721,66 @@ sc_status device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int b
       out->push_back(sc::make_hit(meta, c.x, drift_bins, drift_rate, snr, beam,
                                   coarse_channel, num_timesteps, cand_sum));
   }
+  return SC_OK;
+}
+
+// Checks of a gated search's shape (16 time steps, whole 4096-channel blocks,
+// a network batch that holds them, a compiled fused kernel), and the tile
+// list buffers, allocated on first use.
+sc_status prepare_gate(sc_dd* dd, const sc_pn* pn) {
+  if (!pn) return fail(SC_ERR_INVALID_ARGUMENT, "null proposal network");
+  const int T = dd->rounded, N = dd->num_channels;
+  if (T != sc::pn::kRows || N % sc::pn::kBlockChannels != 0)
+    return fail(SC_ERR_UNSUPPORTED,
+                "the gated search takes 16 time steps and a multiple of 4096 channels");
+  if (N / sc::pn::kBlockChannels > pn->max_batch)
+    return fail(SC_ERR_INVALID_ARGUMENT, "proposal network max_batch below the block count");
+  if (!sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch))
+    return fail(SC_ERR_UNSUPPORTED, "no fused kernel for the current tuning");
+  const int width = sc::fused_tile_width(T, dd->lanes);
+  const long ntiles = ((long)N + width - 1) / width;
+  if (ntiles > (N + 63) / 64 + 1) return fail(SC_ERR_UNSUPPORTED, "tile list too long");
+  if (!dd->gate_list) {
+    sc_status s = dev_alloc(dd, (void**)&dd->gate_list, sizeof(int) * (size_t)((N + 63) / 64 + 1),
+                            "tile list");
+    if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->gate_count, 2 * sizeof(int), "tile count");
+    if (s == SC_OK) s = host_alloc((void**)&dd->h_gate, 2 * sizeof(int), "tile count");
+    if (s != SC_OK) return s;
+  }
+  return SC_OK;
+}
+
+// The gated search up to the tree: column sums and statistics of the whole
+// coarse channel (they do not depend on the search), pooling, the network,
+// and the ascending list of channel tiles that overlap a flagged block.
+sc_status enqueue_gate_front(sc_dd* dd, sc_pn* pn, const void* data, int dtype, long row_stride,
+                             float logit_threshold, hipStream_t stream) {
+  const int T = dd->rounded, N = dd->num_channels;
+  const int nblk = N / sc::pn::kBlockChannels;
+  const int width = sc::fused_tile_width(T, dd->lanes);
+  SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
+  SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
+                                 dd->stats_out, stream),
+         "device stats");
+  SC_HIP(sc::pn::launch_pool(data, dtype, row_stride, nblk, dd->stats_out, pn->z, stream),
+         "pool");
+  SC_HIP(sc::pn::launch_forward(pn->w, pn->z, nblk, pn->a, pn->logits, nullptr, stream),
+         "proposal forward");
+  SC_HIP(sc::pn::launch_tile_list(pn->logits, nblk, logit_threshold, N, width, dd->gate_list,
+                                  dd->gate_count, dd->gate_count + 1, stream),
+         "tile list");
+  return SC_OK;
+}
+
+// The tree over the listed tiles only; unsearched channels keep key 0 (no
+// path), so they never form or suppress a candidate.
+sc_status enqueue_gated_tree(sc_dd* dd, const void* data, int dtype, long row_stride,
+                             const DriftRange& r, hipStream_t stream) {
+  SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, dd->rounded, dd->num_channels, r.bmin,
+                                   r.bmax - r.bmin + 1, dd->lanes, dd->group, dd->prefetch,
+                                   dd->top_key, nullptr, stream, dd->gate_list, dd->gate_count,
+                                   true),
+         "gated tree");
   return SC_OK;
 }
 
@@ -844,62 +910,29 @@ sc_status sc_dd_search_proposed(sc_dd* dd, sc_pn* pn, const void* data, int dtyp
                                 size_t* num_hits, int* flagged_blocks, void* stream_ptr) {
   sc_status s = validate_input(dd, data, dtype, rounded_timesteps, num_channels, row_stride);
   if (s != SC_OK) return s;
-  if (!pn) return fail(SC_ERR_INVALID_ARGUMENT, "null proposal network");
   if (!meta) return fail(SC_ERR_INVALID_ARGUMENT, "null metadata");
   if (!(max_drift > 0) || !std::isfinite(max_drift))
     return fail(SC_ERR_INVALID_ARGUMENT,
                 "max_drift must be > 0 (the reference loops forever at 0, dedoppler.cu:275)");
+  s = prepare_gate(dd, pn);
+  if (s != SC_OK) return s;
   const int T = dd->rounded, N = dd->num_channels;
-  if (T != sc::pn::kRows || N % sc::pn::kBlockChannels != 0)
-    return fail(SC_ERR_UNSUPPORTED,
-                "the gated search takes 16 time steps and a multiple of 4096 channels");
-  const int nblk = N / sc::pn::kBlockChannels;
-  if (nblk > pn->max_batch)
-    return fail(SC_ERR_INVALID_ARGUMENT, "proposal network max_batch below the block count");
-  if (!sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch))
-    return fail(SC_ERR_UNSUPPORTED, "no fused kernel for the current tuning");
   hipStream_t stream = (hipStream_t)stream_ptr;
   const DriftRange r = drift_range(dd, max_drift);
   if (r.window <= 0) return fail(SC_ERR_INVALID_ARGUMENT, "window size is 0");
-  const int nblocks = r.bmax - r.bmin + 1;
-  if ((long)nblocks * T > 0x1FFFFFFFL) return fail(SC_ERR_UNSUPPORTED, "drift range too large");
-  const int width = sc::fused_tile_width(T, dd->lanes);
-  const int ntiles = (N + width - 1) / width;
-  if (!dd->gate_list) {
-    s = dev_alloc(dd, (void**)&dd->gate_list, sizeof(int) * (size_t)((N + 63) / 64 + 1),
-                  "tile list");
-    if (s == SC_OK) s = dev_alloc(dd, (void**)&dd->gate_count, 2 * sizeof(int), "tile count");
-    if (s == SC_OK) s = host_alloc((void**)&dd->h_gate, 2 * sizeof(int), "tile count");
-    if (s != SC_OK) return s;
-  }
-  if (ntiles > (N + 63) / 64 + 1) return fail(SC_ERR_UNSUPPORTED, "tile list too long");
+  if ((long)(r.bmax - r.bmin + 1) * T > 0x1FFFFFFFL)
+    return fail(SC_ERR_UNSUPPORTED, "drift range too large");
   dd->hits.clear();
   dd->state_valid = false;
 
-  // Column sums and statistics of the whole coarse channel first (they do
-  // not depend on the search), then the network, then the full tree on the
-  // flagged blocks' channel tiles only; unsearched channels keep key 0 (no
-  // path), so they never form or suppress a candidate.
   SC_HIP(sc::launch_reset(dd->top_key, N, dd->stats_hist, stream), "reset");
   if (dd->timing) SC_HIP(hipEventRecord(dd->ev_start, stream), "timing event");
-  SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
-  SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
-                                 dd->stats_out, stream),
-         "device stats");
-  SC_HIP(sc::pn::launch_pool(data, dtype, row_stride, nblk, dd->stats_out, pn->z, stream),
-         "pool");
-  SC_HIP(sc::pn::launch_forward(pn->w, pn->z, nblk, pn->a, pn->logits, nullptr, stream),
-         "proposal forward");
-  SC_HIP(sc::pn::launch_tile_list(pn->logits, nblk, logit_threshold, N, width, dd->gate_list,
-                                  dd->gate_count, dd->gate_count + 1, stream),
-         "tile list");
-  SC_HIP(sc::launch_fused_tree_top(data, dtype, row_stride, T, N, r.bmin, nblocks, dd->lanes,
-                                   dd->group, dd->prefetch, dd->top_key, nullptr, stream,
-                                   dd->gate_list, dd->gate_count, true),
-         "gated tree");
+  s = enqueue_gate_front(dd, pn, data, dtype, row_stride, logit_threshold, stream);
+  if (s == SC_OK) s = enqueue_gated_tree(dd, data, dtype, row_stride, r, stream);
+  if (s != SC_OK) return s;
   SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
                                  dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off, dd->cand,
-                                 true, stream),
+                                 true, stream, true),
          "window merge");
   const int nwin = (int)(((long)N + r.window - 1) / r.window);
   SC_HIP(sc::launch_candidate_offsets(data, dtype, row_stride, T, N, nwin, dd->cand, dd->top_off,
@@ -1057,7 +1090,7 @@ sc_status sc_dd_add_incoherent_power(sc_dd* dd, const void* data, int dtype,
 }
 
 sc_status sc_normalize(const float* data, int rounded_timesteps, int num_channels,
-                       long row_stride, int out_dtype, float clamp, void* out, double* mean_out,
+                       long row_stride, int out_dtype, float clamp, void* out, double* loc_out,
                        double* std_out, void* stream_ptr) {
   if (!data || !out) return fail(SC_ERR_INVALID_ARGUMENT, "null pointer");
   if (rounded_timesteps <= 0 || num_channels <= 0 || row_stride < num_channels)
@@ -1069,21 +1102,21 @@ sc_status sc_normalize(const float* data, int rounded_timesteps, int num_channel
   double* moments = nullptr;
   SC_HIP(hipMalloc(&scratch, sc::normalize_scratch_bytes(rounded_timesteps, num_channels)),
          "normalize scratch");
-  hipError_t e = hipMalloc((void**)&moments, 2 * sizeof(double));
+  hipError_t e = hipMalloc((void**)&moments, 3 * sizeof(double));
   if (e != hipSuccess) {
     (void)hipFree(scratch);
     return hip_fail(e, "normalize moments");
   }
   e = sc::launch_normalize(data, rounded_timesteps, num_channels, row_stride, out_dtype, clamp,
                            out, moments, scratch, stream);
-  double h[2] = {0, 0};
+  double h[3] = {0, 0, 0};
   if (e == hipSuccess)
     e = hipMemcpyAsync(h, moments, sizeof(h), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   (void)hipFree(scratch);
   (void)hipFree(moments);
   if (e != hipSuccess) return hip_fail(e, "normalize");
-  if (mean_out) *mean_out = h[0];
+  if (loc_out) *loc_out = h[2];
   if (std_out) *std_out = h[1];
   return SC_OK;
 }
@@ -1156,6 +1189,8 @@ extern "C" sc_status sc_condition_coarse_channel(float* data, int num_timesteps,
 // scratch is large); an instance's path sums wait for the tail of the
 // previous instance on the same handle.
 // Each instance's candidates land in its own pinned host area.
+// A gated batch (with a proposal network) is not captured: it runs as direct
+// launches on the same two streams (enqueue_gated).
 
 struct sc_batch {
   struct Instance {
@@ -1170,10 +1205,19 @@ struct sc_batch {
   std::vector<int> alternates;  // per handle: handles of its shape (1 or 2)
   int4* h_cand = nullptr;
   float* h_stats = nullptr;  // 4 per instance
+  int* h_gate = nullptr;     // gated batch: {tiles searched, flagged blocks} per instance
+  sc_pn* pn = nullptr;       // gated batch: the proposal network (not owned)
+  float logit_threshold = 0.0f;
   hipStream_t capture_stream = nullptr;
   hipStream_t tail_stream = nullptr;
   hipGraphExec_t exec = nullptr;
   int nodes = 0;
+  // Gated batch: the fronts' stream; events {start, done, then front, tree
+  // and tail done per instance}; per instance the previous instance on its
+  // handle (-1 if none).
+  hipStream_t front_stream = nullptr;
+  std::vector<hipEvent_t> gate_events;
+  std::vector<long> prev_on_handle;
 };
 
 namespace {
@@ -1182,22 +1226,93 @@ void batch_release(sc_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   if (b->tail_stream) (void)hipStreamDestroy(b->tail_stream);
+  if (b->front_stream) (void)hipStreamDestroy(b->front_stream);
   for (sc_dd* dd : b->shapes) sc_dd_destroy(dd);
   (void)hipHostFree(b->h_cand);
   (void)hipHostFree(b->h_stats);
+  (void)hipHostFree(b->h_gate);
+  for (hipEvent_t ev : b->gate_events)
+    if (ev) (void)hipEventDestroy(ev);
 }
 
-}  // namespace
+// Gated batch run, as direct launches on three streams joined by events
+// (device-side waits): the trees on one in instance order, each instance's
+// front (reset, column sums, statistics, pooling, network, tile list) on the
+// second, its tail (merge, offsets, copies) on the third. The front of
+// instance i + 1 is enqueued right after tree i, so it runs beside it, once
+// the tail of the previous instance on its handle is done. (Captured into a
+// graph, the same sequence ran each tree only after the previous instance's
+// tail; with the tails on the fronts' stream the fronts became the longer
+// chain.)
+sc_status enqueue_gated(sc_batch* b, hipStream_t stream) {
+  const size_t n = b->inst.size();
+  hipStream_t M = b->capture_stream, F = b->front_stream, Q = b->tail_stream;
+  hipEvent_t* ev = b->gate_events.data();
+  auto front_ev = [&](size_t j) { return ev[2 + 3 * j]; };
+  auto sums_ev = [&](size_t j) { return ev[3 + 3 * j]; };
+  auto tail_ev = [&](size_t j) { return ev[4 + 3 * j]; };
+  SC_HIP(hipEventRecord(ev[0], stream), "batch start");
+  SC_HIP(hipStreamWaitEvent(M, ev[0], 0), "batch start");
+  SC_HIP(hipStreamWaitEvent(F, ev[0], 0), "batch start");
+  SC_HIP(hipStreamWaitEvent(Q, ev[0], 0), "batch start");
+  size_t next_front = 0;
+  // Fronts of the instances below `limit` whose handle is free once the
+  // tails of the instances below `tails` are enqueued.
+  auto issue_fronts = [&](long tails, size_t limit) -> sc_status {
+    for (; next_front < std::min(limit, n) && b->prev_on_handle[next_front] < tails;
+         ++next_front) {
+      sc_batch::Instance& bj = b->inst[next_front];
+      sc_dd* dj = b->shapes[bj.shape];
+      const long prev = b->prev_on_handle[next_front];
+      if (prev >= 0) SC_HIP(hipStreamWaitEvent(F, tail_ev((size_t)prev), 0), "wait tail");
+      SC_HIP(sc::launch_reset(dj->top_key, dj->num_channels, dj->stats_hist, F), "reset");
+      sc_status st = enqueue_gate_front(dj, b->pn, bj.in.data, bj.in.dtype, bj.in.row_stride,
+                                        b->logit_threshold, F);
+      if (st != SC_OK) return st;
+      SC_HIP(hipEventRecord(front_ev(next_front), F), "front done");
+    }
+    return SC_OK;
+  };
+  sc_status s = issue_fronts(0, 1);
+  for (size_t i = 0; i < n && s == SC_OK; ++i) {
+    sc_batch::Instance& bi = b->inst[i];
+    sc_dd* dd = b->shapes[bi.shape];
+    s = issue_fronts((long)i, i + 1);
+    if (s != SC_OK) break;
+    SC_HIP(hipStreamWaitEvent(M, front_ev(i), 0), "wait front");
+    s = enqueue_gated_tree(dd, bi.in.data, bi.in.dtype, bi.in.row_stride, bi.r, M);
+    if (s != SC_OK) break;
+    SC_HIP(hipEventRecord(sums_ev(i), M), "tree done");
+    s = issue_fronts((long)i, i + 2);
+    if (s != SC_OK) break;
+    SC_HIP(hipStreamWaitEvent(Q, sums_ev(i), 0), "wait tree");
+    s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, bi.in.data, bi.in.dtype,
+                            bi.in.row_stride, Q, b->h_cand + bi.cand_offset, b->h_stats + 4 * i,
+                            true, true);
+    if (s != SC_OK) break;
+    SC_HIP(hipMemcpyAsync(b->h_gate + 2 * i, dd->gate_count, 2 * sizeof(int),
+                          hipMemcpyDeviceToHost, Q),
+           "copy tile count");
+    SC_HIP(hipEventRecord(tail_ev(i), Q), "tail done");
+  }
+  if (s != SC_OK) return s;
+  // Tail i follows tree i and front i precedes it: the tails' stream ends last.
+  SC_HIP(hipEventRecord(ev[1], Q), "batch done");
+  SC_HIP(hipStreamWaitEvent(stream, ev[1], 0), "batch done");
+  return SC_OK;
+}
 
-extern "C" {
-
-sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuning* tuning,
-                          int num_tuning, sc_batch** out) {
+sc_status batch_create(const sc_instance* instances, int count, const sc_tuning* tuning,
+                       int num_tuning, sc_pn* pn, float logit_threshold, sc_batch** out) {
   if (!out) return fail(SC_ERR_INVALID_ARGUMENT, "null output handle");
   *out = nullptr;
   if (!instances || count <= 0) return fail(SC_ERR_INVALID_ARGUMENT, "empty batch");
   if (num_tuning > 0 && !tuning) return fail(SC_ERR_INVALID_ARGUMENT, "null tuning table");
+  if (pn && !std::isfinite(logit_threshold) && !(logit_threshold < 0))
+    return fail(SC_ERR_INVALID_ARGUMENT, "logit_threshold must not be NaN or +inf");
   sc_batch* b = new sc_batch();
+  b->pn = pn;
+  b->logit_threshold = logit_threshold;
   sc_status s = SC_OK;
   size_t total_windows = 0;
   for (int i = 0; i < count && s == SC_OK; ++i) {
@@ -1258,6 +1373,13 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
       s = fail(SC_ERR_UNSUPPORTED, "instance " + std::to_string(i) + ": drift range");
       break;
     }
+    if (pn) {
+      s = prepare_gate(b->shapes[shape], pn);
+      if (s != SC_OK) {
+        s = fail(s, "instance " + std::to_string(i) + ": " + sc::last_error());
+        break;
+      }
+    }
     bi.cand_offset = total_windows;
     total_windows += ((size_t)in.num_channels + bi.r.window - 1) / bi.r.window;
     b->inst.push_back(bi);
@@ -1265,9 +1387,13 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
   if (s == SC_OK) s = host_alloc((void**)&b->h_cand, total_windows * sizeof(int4), "candidates");
   if (s == SC_OK)
     s = host_alloc((void**)&b->h_stats, 4 * sizeof(float) * (size_t)count, "statistics");
+  if (s == SC_OK && pn)
+    s = host_alloc((void**)&b->h_gate, 2 * sizeof(int) * (size_t)count, "tile counts");
   if (s == SC_OK) {
     hipError_t e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->tail_stream, hipStreamNonBlocking);
+    if (e == hipSuccess && pn)
+      e = hipStreamCreateWithFlags(&b->front_stream, hipStreamNonBlocking);
     if (e != hipSuccess) s = hip_fail(e, "stream");
   }
   std::vector<hipEvent_t> events;
@@ -1276,7 +1402,26 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
     if (e == hipSuccess) events.push_back(*ev);
     return e;
   };
-  if (s == SC_OK) {
+  if (s == SC_OK && pn) {
+    // Gated batch: run as direct launches (enqueue_gated); its events and the
+    // previous instance on each handle are set up here.
+    b->prev_on_handle.assign(b->inst.size(), -1);
+    std::vector<long> last(b->shapes.size(), -1);
+    for (size_t j = 0; j < b->inst.size(); ++j) {
+      b->prev_on_handle[j] = last[b->inst[j].shape];
+      last[b->inst[j].shape] = (long)j;
+    }
+    b->gate_events.assign(2 + 3 * b->inst.size(), nullptr);
+    for (hipEvent_t& ev : b->gate_events) {
+      hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e != hipSuccess) {
+        ev = nullptr;
+        s = hip_fail(e, "event");
+        break;
+      }
+    }
+  }
+  if (s == SC_OK && !pn) {
     hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) s = hip_fail(e, "begin capture");
     // last_tail[h]: the tail event of the previous instance on handle h.
@@ -1332,10 +1477,31 @@ sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuni
   return SC_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+sc_status sc_batch_create(const sc_instance* instances, int count, const sc_tuning* tuning,
+                          int num_tuning, sc_batch** out) {
+  return batch_create(instances, count, tuning, num_tuning, nullptr, 0.0f, out);
+}
+
+sc_status sc_batch_create_proposed(const sc_instance* instances, int count,
+                                   const sc_tuning* tuning, int num_tuning, sc_pn* pn,
+                                   float logit_threshold, sc_batch** out) {
+  if (!pn) return fail(SC_ERR_INVALID_ARGUMENT, "null proposal network");
+  return batch_create(instances, count, tuning, num_tuning, pn, logit_threshold, out);
+}
+
 sc_status sc_batch_run(sc_batch* b, void* stream_ptr) {
   if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
   hipStream_t stream = (hipStream_t)stream_ptr;
-  SC_HIP(hipGraphLaunch(b->exec, stream), "batch graph launch");
+  if (b->pn) {
+    const sc_status s = enqueue_gated(b, stream);
+    if (s != SC_OK) return s;
+  } else {
+    SC_HIP(hipGraphLaunch(b->exec, stream), "batch graph launch");
+  }
   SC_HIP(hipStreamSynchronize(stream), "batch");
   for (size_t i = 0; i < b->inst.size(); ++i) {
     sc_batch::Instance& bi = b->inst[i];
@@ -1358,6 +1524,17 @@ sc_status sc_batch_copy_hits(const sc_batch* b, int instance, sc_hit* hits, size
   const std::vector<sc_hit>& h = b->inst[instance].hits;
   if (num_hits) *num_hits = h.size();
   if (hits && capacity) std::memcpy(hits, h.data(), std::min(capacity, h.size()) * sizeof(sc_hit));
+  return SC_OK;
+}
+
+sc_status sc_batch_flagged_blocks(const sc_batch* b, int instance, int* tiles_searched,
+                                  int* flagged_blocks) {
+  if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
+  if (!b->pn) return fail(SC_ERR_INVALID_ARGUMENT, "not a gated batch");
+  if (instance < 0 || instance >= (int)b->inst.size())
+    return fail(SC_ERR_INVALID_ARGUMENT, "instance out of range");
+  if (tiles_searched) *tiles_searched = b->h_gate[2 * instance];
+  if (flagged_blocks) *flagged_blocks = b->h_gate[2 * instance + 1];
   return SC_OK;
 }
 

@@ -126,7 +126,7 @@ hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsign
 hipError_t launch_window_merge(const top_key_t* key, int N, int T, int bmin, int window,
                                double snr_threshold, const float* stats, float* top_sum,
                                int* top_blk, int* top_off, int4* cand_out, bool block_keys,
-                               hipStream_t stream);
+                               hipStream_t stream, bool skip_empty = false);
 // After a block-key merge (T_r <= 16): each candidate's path offset, from the
 // input of the search.
 hipError_t launch_candidate_offsets(const void* x, int dtype, long stride, int T, int N,
@@ -138,7 +138,7 @@ hipError_t launch_gather(const float* sums, int N, const int* offsets, const int
 
 // ---- normalisation and synthetic data -------------------------------------
 hipError_t launch_normalize(const float* x, int T, int N, long stride, int out_dtype,
-                            float clamp, void* out, double* moments /* device, 2 */,
+                            float clamp, void* out, double* moments /* device: mean, std, location */,
                             void* scratch, hipStream_t stream);
 size_t normalize_scratch_bytes(int T, int N);
 

@@ -107,9 +107,15 @@ __global__ void __launch_bounds__(kWave)
   const unsigned long long group_mask = kLanes == kWave ? ~0ull : ((1ull << kLanes) - 1);
   int4 r = make_int4(-1, 0, 0, 0);
   int pmax = 0;
+  // A candidate whose top sum is the zero the search starts from (no positive
+  // path; only when the threshold is below zero) keeps the block and offset
+  // the merge read from the top arrays, as the reference reports them
+  // (dedoppler.cu:305-306).
+  bool empty = false;
   if (w < nwin) {
     r = cand[w];
-    if (r.x >= 0) {
+    empty = r.x >= 0 && r.y == 0;
+    if (r.x >= 0 && !empty) {
       const int last0 = r.x + (T - 1) * r.z;
       pmax = last0 < 0 ? 0 : min(T, N - last0);
     }
@@ -122,7 +128,7 @@ __global__ void __launch_bounds__(kWave)
     const unsigned long long mine = (__ballot(match) >> (slot * kLanes)) & group_mask;
     if (p_best < 0 && mine) p_best = p0 + __ffsll((long long)mine) - 1;
   }
-  if (lane == 0 && w < nwin && r.x >= 0) {
+  if (lane == 0 && w < nwin && r.x >= 0 && !empty) {
     // -1: no offset reproduces the recorded sum (the tree kernels and this
     // re-addition disagree on an addition order); the host reports an error.
     r.w = p_best;
@@ -1060,22 +1066,49 @@ __device__ __forceinline__ void stats_ranks(int N, unsigned (&want)[3]) {
   want[2] = (unsigned)rank_last(N);
 }
 
-// Counts `sel(key)` into an LDS histogram for every value passing `match`,
-// then flushes the non-zero bins into global memory.
+// Counts `digit(key)` into an LDS histogram for every value passing `match`,
+// then flushes the non-zero bins into global memory. Two 16-bit counters per
+// LDS word (3 * NBINS / 2 words: 12 KB at 2048 bins), so the workgroup fits
+// beside the tree kernels' 20 KB windows when one of them retires; the counts
+// are flushed every 255 strides (at most 256 * 255 increments of one counter
+// in between), which only happens above 16.7 M values.
 template <int NBINS, typename Match, typename Digit>
 __device__ void block_histogram(const float* __restrict__ v, int N, unsigned* lds,
                                 unsigned* __restrict__ out, Match match, Digit digit) {
-  for (int i = threadIdx.x; i < 3 * NBINS; i += blockDim.x) lds[i] = 0;
+  constexpr int kWords = 3 * NBINS / 2;
+  for (int i = threadIdx.x; i < kWords; i += blockDim.x) lds[i] = 0;
   __syncthreads();
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    const unsigned k = ord_key(v[i]);
+  const long stride = (long)gridDim.x * blockDim.x;
+  int strides = 0;
+  for (long base = (long)blockIdx.x * blockDim.x; base < N; base += stride) {
+    const long i = base + threadIdx.x;
+    if (i < N) {
+      const unsigned k = ord_key(v[i]);
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-      if (match(r, k)) atomicAdd(&lds[r * NBINS + digit(k)], 1u);
+      for (int r = 0; r < 3; ++r)
+        if (match(r, k)) {
+          const unsigned b = (unsigned)(r * NBINS) + digit(k);
+          atomicAdd(&lds[b >> 1], 1u << (16 * (b & 1)));
+        }
+    }
+    if (++strides == 255 && base + stride < N) {  // uniform over the workgroup
+      __syncthreads();
+      for (int w = threadIdx.x; w < kWords; w += blockDim.x) {
+        const unsigned x = lds[w];
+        if (x & 0xFFFFu) atomicAdd(&out[2 * w], x & 0xFFFFu);
+        if (x >> 16) atomicAdd(&out[2 * w + 1], x >> 16);
+        lds[w] = 0;
+      }
+      __syncthreads();
+      strides = 0;
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 3 * NBINS; i += blockDim.x)
-    if (lds[i]) atomicAdd(&out[i], lds[i]);
+  for (int w = threadIdx.x; w < kWords; w += blockDim.x) {
+    const unsigned x = lds[w];
+    if (x & 0xFFFFu) atomicAdd(&out[2 * w], x & 0xFFFFu);
+    if (x >> 16) atomicAdd(&out[2 * w + 1], x >> 16);
+  }
 }
 
 __global__ void __launch_bounds__(kStatsThreads)
@@ -1123,7 +1156,7 @@ __device__ void select_l2(const unsigned* __restrict__ hist, unsigned (&pre)[3],
 
 __global__ void __launch_bounds__(kStatsThreads)
     stats_pass2_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
-  __shared__ unsigned h[3 * kL2];
+  __shared__ unsigned h[3 * kL2 / 2];
   unsigned pre[3], want[3], below[3];
   select_l1(hist, N, pre, want, below);
   const unsigned p0 = pre[0], p1 = pre[1], p2 = pre[2];
@@ -1138,7 +1171,7 @@ __global__ void __launch_bounds__(kStatsThreads)
 
 __global__ void __launch_bounds__(kStatsThreads)
     stats_pass3_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
-  __shared__ unsigned h[3 * kL3];
+  __shared__ unsigned h[3 * kL3 / 2];
   unsigned pre[3], want[3], below[3];
   select_l1(hist, N, pre, want, below);
   select_l2(hist, pre, want, below);
@@ -1270,7 +1303,11 @@ __global__ void __launch_bounds__(256)
     window_merge_kernel(const top_key_t* __restrict__ key, int N, int T, int bmin, int window,
                         double snr_threshold, const float* __restrict__ stats,
                         float* __restrict__ top_sum, int* __restrict__ top_blk,
-                        int* __restrict__ top_off, int4* __restrict__ out, int block_keys) {
+                        int* __restrict__ top_off, int4* __restrict__ out, int flags) {
+  const bool block_keys = flags & 1;
+  // Gated search: a channel without a key (not searched, or no positive
+  // path) neither forms nor suppresses a candidate.
+  const float empty = (flags & 2) ? -INFINITY : 0.0f;
   __shared__ float sv[4];
   __shared__ int si[4];
   __shared__ int any_larger;
@@ -1288,6 +1325,7 @@ __global__ void __launch_bounds__(256)
   for (long f = lo + tid; f < hi; f += blockDim.x) {
     const top_key_t k = key[f];
     const float v = k ? (block_keys ? block_key_sum(k) : key_sum(k)) : 0.0f;
+    const float cv = k ? v : empty;
     if (k && block_keys) {
       top_blk[f] = (int)~(unsigned)k + bmin;
       top_off[f] = -1;  // completed for candidates by candidate_offsets_kernel
@@ -1297,8 +1335,8 @@ __global__ void __launch_bounds__(256)
       top_off[f] = (int)(code % (unsigned)T);
     }
     top_sum[f] = v;
-    if (v > bv) {
-      bv = v;
+    if (cv > bv) {
+      bv = cv;
       bi = (int)f;
     }
   }
@@ -1336,7 +1374,7 @@ __global__ void __launch_bounds__(256)
   int larger = 0;
   for (long f = s0 + tid; f < s1; f += blockDim.x) {
     const top_key_t k = key[f];
-    larger |= (k ? (block_keys ? block_key_sum(k) : key_sum(k)) : 0.0f) > bv;
+    larger |= (k ? (block_keys ? block_key_sum(k) : key_sum(k)) : empty) > bv;
   }
   if (larger) any_larger = 1;
   __syncthreads();
@@ -1414,10 +1452,16 @@ __device__ __forceinline__ __hip_bfloat16 from_f32<__hip_bfloat16>(float v) {
 template <>
 __device__ __forceinline__ float from_f32<float>(float v) { return v; }
 
+// Location of the normalisation: the median column sum over T (stats_out[0]
+// from the device statistics of the raw column sums).
+__global__ void norm_location_kernel(const float* stats_out, int T, double* moments) {
+  if (threadIdx.x == 0) moments[2] = (double)stats_out[0] / (double)T;
+}
+
 template <typename Out>
 __global__ void norm_apply_kernel(const float* __restrict__ x, int T, int N, long stride,
                                   const double* moments, float clamp, Out* __restrict__ out) {
-  const double m = moments[0];
+  const double m = moments[2];
   const double sd = moments[1] > 0.0 ? moments[1] : 1.0;
   const long total = (long)T * N;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total;
@@ -1880,11 +1924,11 @@ hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsign
 hipError_t launch_window_merge(const top_key_t* key, int N, int T, int bmin, int window,
                                double snr_threshold, const float* stats, float* top_sum,
                                int* top_blk, int* top_off, int4* cand_out, bool block_keys,
-                               hipStream_t stream) {
+                               hipStream_t stream, bool skip_empty) {
   const int nwin = (int)(((long)N + window - 1) / window);
   hipLaunchKernelGGL(window_merge_kernel, dim3(nwin), dim3(256), 0, stream, key, N, T, bmin,
                      window, snr_threshold, stats, top_sum, top_blk, top_off, cand_out,
-                     (int)block_keys);
+                     (block_keys ? 1 : 0) | (skip_empty ? 2 : 0));
   return hipGetLastError();
 }
 
@@ -1896,12 +1940,44 @@ hipError_t launch_gather(const float* sums, int N, const int* offsets, const int
   return hipGetLastError();
 }
 
-size_t normalize_scratch_bytes(int /*T*/, int /*N*/) { return sizeof(double) * kNormBlocks; }
+namespace {
+// Scratch of launch_normalize: partial sums, column sums, statistics state,
+// histograms, statistics output.
+constexpr size_t norm_align(size_t n) { return (n + 255) / 256 * 256; }
+struct NormScratch {
+  size_t part, colsum, state, hist, stats, total;
+  explicit NormScratch(int N) {
+    part = 0;
+    colsum = norm_align(sizeof(double) * kNormBlocks);
+    state = colsum + norm_align(sizeof(float) * (size_t)N);
+    hist = state + norm_align(device_stats_scratch_bytes(N));
+    stats = hist + norm_align(device_stats_hist_bytes());
+    total = stats + norm_align(4 * sizeof(float));
+  }
+};
+}  // namespace
+
+size_t normalize_scratch_bytes(int /*T*/, int N) { return NormScratch(N).total; }
 
 hipError_t launch_normalize(const float* x, int T, int N, long stride, int out_dtype, float clamp,
                             void* out, double* moments, void* scratch, hipStream_t stream) {
-  double* part = (double*)scratch;
+  const NormScratch L(N);
+  char* base = (char*)scratch;
+  double* part = (double*)(base + L.part);
   const double count = (double)T * (double)N;
+  // Location: the median of the raw column sums over T, so the normalised
+  // column sums have median ~0 whatever share of the power narrowband RFI
+  // holds (the mean would put them far below zero): every positive SNR
+  // threshold then lies above the zero the top path sums start from.
+  float* colsum = (float*)(base + L.colsum);
+  unsigned* hist = (unsigned*)(base + L.hist);
+  float* stats = (float*)(base + L.stats);
+  hipError_t e = launch_colsum(x, 0, stride, T, N, colsum, stream);
+  if (e == hipSuccess) e = launch_reset(nullptr, 0, hist, stream);
+  if (e == hipSuccess)
+    e = launch_device_stats(colsum, N, base + L.state, hist, stats, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(norm_location_kernel, dim3(1), dim3(64), 0, stream, stats, T, moments);
   hipLaunchKernelGGL(norm_sum_kernel, dim3(kNormBlocks), dim3(256), 0, stream, x, T, N, stride,
                      part);
   hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(64), 0, stream, part, kNormBlocks, count,

@@ -197,6 +197,11 @@ extern "C" sc_status sc_dd_search_static(int num_timesteps, int rounded_timestep
     for (int i = 0; i < n; ++i) {
       const int4 c = buf[i];
       if (c.x < 0) continue;
+      if (c.w < 0 && c.y == 0)
+        return fail(SC_ERR_UNSUPPORTED,
+                    "the threshold is below zero and channel " + std::to_string(c.x) +
+                        " has no positive path: its hit takes the path offset of an earlier "
+                        "search (dedoppler.cu:305-306), which this search does not keep");
       if (c.w < 0)
         return fail(SC_ERR_INTERNAL, "no path offset reproduces the top sum at channel " +
                                          std::to_string(c.x));

@@ -88,7 +88,9 @@ __global__ void __launch_bounds__(256)
 // row%4. The product is formed transposed, D[channel][position] =
 // W[channel][k] * X[k][position], so each lane ends with 4 consecutive
 // channels of one position: one 8-byte store per lane and channel tile.
-template <int CIN, int COUT, int SH, int SW>
+// ROWS output rows per workgroup (blockIdx.y * ROWS ..), one after the other,
+// so the weights held in registers are loaded once for all of them.
+template <int CIN, int COUT, int SH, int SW, int ROWS>
 __global__ void __launch_bounds__(64)
     conv_fwd_kernel(const unsigned short* __restrict__ in, int HI, int WI,
                     const unsigned short* __restrict__ wb, const float* __restrict__ bias,
@@ -100,7 +102,6 @@ __global__ void __launch_bounds__(64)
   const int lane = threadIdx.x;
   const int r = lane % 16, g = lane / 16;
   const int wo0 = blockIdx.x * 64;
-  const int ho = blockIdx.y;
   const long b = blockIdx.z;
 
   bf16x4 bw[TAPS][KS][NT];
@@ -115,6 +116,13 @@ __global__ void __launch_bounds__(64)
         bw[t][s][n] = *(const bf16x4*)w;
       }
 
+  float bsv[NT][4];
+#pragma unroll
+  for (int n = 0; n < NT; ++n)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bsv[n][i] = bias[n * 16 + 4 * g + i];
+
+  for (int ho = blockIdx.y * ROWS; ho < min(HO, (int)(blockIdx.y + 1) * ROWS); ++ho) {
   f32x4 acc[4][NT];
 #pragma unroll
   for (int m = 0; m < 4; ++m)
@@ -162,17 +170,15 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
     const int ch0 = n * 16 + 4 * g;
-    float bsv[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) bsv[i] = bias[ch0 + i];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const int pos = wo0 + 16 * m + r;
       bf16x4 v;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = (short)f2bf(fmaxf(acc[m][n][i] + bsv[i], 0.0f));
+      for (int i = 0; i < 4; ++i) v[i] = (short)f2bf(fmaxf(acc[m][n][i] + bsv[n][i], 0.0f));
       *(bf16x4*)(out + ((b * HO + ho) * WO + pos) * COUT + ch0) = v;
     }
+  }
   }
 }
 
@@ -476,11 +482,13 @@ hipError_t launch_pool(const void* x, int dtype, long stride, int nblk, const fl
 
 hipError_t launch_forward(const Weights& w, const unsigned short* z, int B, Acts& a,
                           float* logits, int* argmax, hipStream_t stream) {
-  hipLaunchKernelGGL((conv_fwd_kernel<1, kC1, 1, 2>), dim3(kW1 / 64, kH1, B), dim3(64), 0,
+  // conv1's weights are one register tile: a row per workgroup; conv2 and
+  // conv3 (15 and 30 KB of weights per workgroup) take all their rows.
+  hipLaunchKernelGGL((conv_fwd_kernel<1, kC1, 1, 2, 1>), dim3(kW1 / 64, kH1, B), dim3(64), 0,
                      stream, z, kRows, kPooled, w.w1b, w.params + kOffB1, a.h1, kH1, kW1);
-  hipLaunchKernelGGL((conv_fwd_kernel<kC1, kC2, 2, 2>), dim3(kW2 / 64, kH2, B), dim3(64), 0,
+  hipLaunchKernelGGL((conv_fwd_kernel<kC1, kC2, 2, 2, kH2>), dim3(kW2 / 64, 1, B), dim3(64), 0,
                      stream, a.h1, kH1, kW1, w.w2b, w.params + kOffB2, a.h2, kH2, kW2);
-  hipLaunchKernelGGL((conv_fwd_kernel<kC2, kC3, 2, 2>), dim3(kW3 / 64, kH3, B), dim3(64), 0,
+  hipLaunchKernelGGL((conv_fwd_kernel<kC2, kC3, 2, 2, kH3>), dim3(kW3 / 64, 1, B), dim3(64), 0,
                      stream, a.h2, kH2, kW2, w.w3b, w.params + kOffB3, a.h3, kH3, kW3);
   hipLaunchKernelGGL(head_kernel, dim3(B), dim3(256), 0, stream, a.h3, w.params, logits, argmax);
   return hipGetLastError();

@@ -1,5 +1,6 @@
 """Reduced-precision storage: FP16/BF16 samples after per-coarse-channel
-normalisation (FP64 mean/deviation on the GPU), FP32 tree accumulation.
+normalisation on the GPU ((x - median column sum / T) / FP64 deviation), FP32
+tree accumulation.
 
 * Tree parity: the half-width search equals the oracle run on the same
   samples widened to float32, bit for bit (the tree converts each sample once
@@ -75,11 +76,13 @@ def test_half_storage_parity_longer_windows(gpu, oracle, dtype, T_r):
 def test_half_storage_tree_parity(gpu, oracle, dtype):
     import torch
     x = _rfi_waterfall(torch, 3)
-    xn, mean, sd = sc.normalize(x, dtype)
+    xn, loc, sd = sc.normalize(x, dtype)
     widened = xn.float().cpu().numpy()
-    # The FP64 moments match numpy's over the same samples.
-    ref = x.double().cpu().numpy()
-    assert abs(mean - ref.mean()) <= 1e-9 * max(1.0, abs(ref.mean()))
+    # loc is the median (rank N/2) of the raw column sums, summed in the
+    # reference's order, over T; std the FP64 deviation of all samples.
+    raw = x.cpu().numpy()
+    assert loc == float(np.sort(oracle.sum_columns(raw))[N // 2]) / T
+    ref = raw.astype(np.float64)
     assert abs(sd - ref.std()) <= 1e-9 * ref.std()
     dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
     hits = []
@@ -91,6 +94,11 @@ def test_half_storage_tree_parity(gpu, oracle, dtype):
     np.testing.assert_array_equal(st["top_sums"], want.top_sums)
     assert [(h.index, h.drift_steps, h.snr) for h in hits] == \
         [(h["index"], h["drift_steps"], h["snr"]) for h in want.hits]
+    # The normalised column sums are centred on zero, so the SNR threshold is a
+    # positive path sum. (Around the mean, which the RFI pulls up, the median
+    # column sum would lie this many deviations below zero:)
+    print("mean-centred median: %.1f deviations" % (-T * (ref.mean() - loc) / sd / st["std"]))
+    assert abs(st["median"]) <= 0.05 * st["std"]
 
 
 @pytest.mark.parametrize("dtype", [sc.SC_F16, sc.SC_BF16])

@@ -7,7 +7,8 @@ rounding boundary may round the other way), deeper ones identical in >= 99 %
 of elements and within 2^-7 relative + 1 % of the layer's RMS;
 logits within 1e-2; the gradient within 3 % relative L2 error of the oracle's
 (same argmax positions); the gated search with every block flagged equals the
-plain device-statistics search hit for hit.
+plain device-statistics search hit for hit; the gated batch equals the per-call
+gated search on every field (and the plain batch with every block flagged).
 """
 
 import numpy as np
@@ -130,6 +131,68 @@ def test_gated_search_with_every_block_flagged_equals_plain(gpu):
     none = []
     assert dd.search_proposed(buf, net, meta, -1, 0, 10.0, 0.0, 10.0, 1e30, none) == 0
     assert none == []
+
+
+def _fields(hits):
+    return [(h.index, h.drift_steps, h.drift_rate, h.snr, h.power, h.frequency, h.beam,
+             h.coarse_channel, h.num_timesteps) for h in hits]
+
+
+def test_gated_batch_equals_per_call_gated_search(gpu):
+    # The gated batch (one graph; each coarse channel's statistics, network
+    # and tile list on a chain of their own beside the previous channel's
+    # tree) against the per-call gated search of each coarse channel: the
+    # same tiles searched, the same blocks flagged, every hit field equal -
+    # FP32 and BF16 storage mixed in one batch, two runs in a row. Below
+    # every logit it is the plain batch, hit for hit.
+    import torch
+    from seticore_amd import batch as B
+    N, n_cc = 1 << 18, 6
+    nblk = N // 4096
+    net = sc.ProposalNet(max_batch=nblk, seed=5)
+    xs = [_coarse_channel(torch, N, 40 + i) for i in range(n_cc)]
+    dtypes = [sc.SC_F32 if i % 2 == 0 else sc.SC_BF16 for i in range(n_cc)]
+    data = [x if d == sc.SC_F32 else sc.normalize(x, sc.SC_BF16)[0] for x, d in zip(xs, dtypes)]
+    meta = sc.FilterbankMetadata(coarse_channel_size=N, foff=FOFF)
+    dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
+    dd.set_stats_mode(sc.SC_STATS_DEVICE)
+    # A threshold that flags about half of the blocks: the median logit.
+    logits = []
+    for x in xs:
+        dd.search(sc.FilterbankBuffer(T, N, data=x), meta, -1, 0, 10.0, 0.0, 10.0, [])
+        st = dd.state()
+        stats = torch.tensor([st["median"], st["std"]], dtype=torch.float32, device="cuda")
+        logits.append(net.forward(net.pool(x, stats)).cpu())
+    thr = float(torch.cat(logits).median())
+    bufs = [sc.FilterbankBuffer(T, N, data=d, dtype=t) for d, t in zip(data, dtypes)]
+    inst = [B.instance(d, T, FOFF, TSAMP, 10.0, 0.0, 10.0, coarse_channel=i, dtype=t,
+                       metadata=meta) for i, (d, t) in enumerate(zip(data, dtypes))]
+    per_call = []
+    for i in range(n_cc):
+        out = []
+        flagged = dd.search_proposed(bufs[i], net, meta, -1, i, 10.0, 0.0, 10.0, thr, out)
+        per_call.append((flagged, out))
+    flagged_total = sum(f for f, _ in per_call)
+    print("flagged %d of %d blocks, %d hits" % (flagged_total, n_cc * nblk,
+                                                sum(len(o) for _, o in per_call)))
+    assert 0 < flagged_total < n_cc * nblk
+    gb = B.Batch(inst, proposal=net, logit_threshold=thr)
+    for _ in range(2):
+        gb.run()
+        for i, (flagged, out) in enumerate(per_call):
+            assert gb.flagged(i)[1] == flagged, i
+            assert _fields(gb.hits(i)) == _fields(out), i
+    assert any(out for _, out in per_call)
+    # Every block flagged: the plain batch's hits.
+    everything = B.Batch(inst, proposal=net, logit_threshold=-1e30)
+    plain = B.Batch(inst)
+    everything.run()
+    plain.run()
+    for i in range(n_cc):
+        assert everything.flagged(i)[1] == nblk
+        assert _fields(everything.hits(i)) == _fields(plain.hits(i)), i
+    with pytest.raises(sc.SeticoreError):
+        plain.flagged(0)
 
 
 def test_trained_network_beats_the_brightest_pixel_heuristic(gpu):
