@@ -8,6 +8,7 @@ import pytest
 
 import seticore_amd as sc
 from seticore_amd import batch as B
+from parity_helpers import device_hits_match_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -104,11 +105,13 @@ def test_autotuned_batch_equals_separate_searches(gpu):
     assert _batch_hits(b) == want
 
 
-def test_multibeam_regions_in_place(gpu):
+def test_multibeam_regions_in_place(gpu, oracle):
     # The beamformer feed (beamforming_pipeline.cpp:230-245): every (beam,
     # coarse channel) region of a multibeam buffer searched in place through
     # its row stride equals searching a contiguous copy of the region
-    # (MultibeamBuffer::copyRegionAsync, multibeam_buffer.cu:65-76).
+    # (MultibeamBuffer::copyRegionAsync, multibeam_buffer.cu:65-76), and the
+    # oracle's search of that region (statistics differences only between the
+    # two thresholds, every common hit bit-equal).
     import torch
     beams, T, ccs, ncc = 3, 16, 1 << 15, 4
     multibeam = torch.empty((beams, T, ncc * ccs), dtype=torch.float32, device="cuda")
@@ -125,7 +128,7 @@ def test_multibeam_regions_in_place(gpu):
     b.run()
     dd = sc.Dedopplerer(T, ccs, FOFF, TSAMP)
     dd.set_stats_mode(sc.SC_STATS_DEVICE)
-    total = 0
+    total = same = 0
     for i, it in enumerate(inst):
         beam, cc = divmod(i, ncc)
         region = multibeam[beam, :, cc * ccs:(cc + 1) * ccs].contiguous()
@@ -135,5 +138,10 @@ def test_multibeam_regions_in_place(gpu):
         assert [h.key() + (h.snr, h.frequency) for h in got] == \
             [h.key() + (h.snr, h.frequency) for h in want]
         assert all(h.beam == beam for h in got)
+        data = region.cpu().numpy()
+        ref = oracle.search(data, T, FOFF, TSAMP, 2.0, 0.0, 8.0, fch1=8000.0,
+                            coarse_channel=cc, beam=beam, num_threads=8)
+        summary = device_hits_match_oracle(got, ref, data, 8.0, T - 1)
+        same += summary["same"]
         total += len(got)
-    assert total > 0
+    assert total > 0 and same >= total - 2
