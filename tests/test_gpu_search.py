@@ -331,6 +331,47 @@ def test_stale_blocks_quirk(gpu, oracle):
     np.testing.assert_array_equal(s2["top_offsets"], s1["top_offsets"])
 
 
+def test_threshold_below_zero(gpu, oracle):
+    # dedoppler.cu:205 starts the top sums at zero and dedoppler.cu:273-317
+    # compares them with snr * std + median. With negative power and that
+    # threshold below zero, channels without a positive path become candidates
+    # reported with whatever the block and offset arrays hold - zeros from
+    # creation here, as in the oracle. Host-exact: every field bit-equal.
+    # Device statistics on a fresh handle: the same hits (SNR to the FP64
+    # tolerance). After a block-key search has left those channels without an
+    # offset: SC_ERR_UNSUPPORTED, not a made-up drift. A gated search skips
+    # channels without a key.
+    from parity_helpers import host_exact_hits_equal
+    T, N, foff, tsamp = 16, 8192, -2.79e-6, 18.25
+    data = -_waterfall(T, N, 11)
+    hits, _, ref = _compare_search(oracle, data, T, foff, tsamp, 1.0, 0.0, 2.0)
+    assert ref.median + 2.0 * ref.std < 0 and len(ref.hits) > 0
+    assert all(h["power"] == 0.0 for h in ref.hits)
+    host_exact_hits_equal(hits, ref.hits)
+    dev, _, _ = _compare_search(oracle, data, T, foff, tsamp, 1.0, 0.0, 2.0,
+                                stats_mode=sc.SC_STATS_DEVICE)
+    assert [(h.index, h.drift_steps, h.power, h.frequency) for h in dev] == \
+        [(h["index"], h["drift_steps"], h["power"], h["frequency"]) for h in ref.hits]
+    med64, _, std64 = oracle.stats_fp64(data)
+    snr64 = (0.0 - med64) / std64
+    assert all(abs(h.snr - snr64) <= 1e-4 * abs(snr64) for h in dev)
+    # A block-key search first (every channel has a positive path; nothing
+    # crosses the threshold, so no offset is kept), then the negative input.
+    dd = sc.Dedopplerer(T, N, foff, tsamp)
+    dd.set_stats_mode(sc.SC_STATS_DEVICE)
+    meta = sc.FilterbankMetadata(coarse_channel_size=N, foff=foff)
+    dd.search(sc.FilterbankBuffer(T, N, data=_to_gpu(-data)), meta, -1, 0, 1.0, 0.0, 1e9, [])
+    with pytest.raises(sc.SeticoreError) as err:
+        dd.search(sc.FilterbankBuffer(T, N, data=_to_gpu(data)), meta, -1, 0, 1.0, 0.0, 2.0, [])
+    assert err.value.status == sc.SC_ERR_UNSUPPORTED
+    # Gated, every block flagged: no channel has a key, so no hit.
+    net = sc.ProposalNet(max_batch=N // 4096, seed=1)
+    gated = []
+    flagged = dd.search_proposed(sc.FilterbankBuffer(T, N, data=_to_gpu(data)), net, meta, -1, 0,
+                                 1.0, 0.0, 2.0, -1e30, gated)
+    assert flagged == N // 4096 and gated == []
+
+
 def test_config1_full_size_parity(gpu, oracle):
     # One coarse channel at the benchmark's CPU-runnable size: 16 x 2^20, 8 tones,
     # max drift 4 Hz/s, SNR threshold 10.
