@@ -92,7 +92,7 @@ sc_status sc_dd_set_stats_mode(sc_dd* dd, int mode);
 /* Shape of the one-pass tree kernel used when rounded_timesteps <= 256 (the
  * default; selecting a shape undoes sc_dd_set_two_pass). At 64..256 (level-16
  * trees staged in LDS) the shape is (4, drift blocks per workgroup in
- * {1, 2, 4, 8, 16, 32}, 0); at 128 and 256 also (4, G, 1), one workgroup per
+ * {1, 2, 4, 8, 16, 32, 64, 128, 256}, 0); at 128 and 256 also (4, G, 1), one workgroup per
  * group of offsets instead of one for all. At 32 the same kernel is (4, G, 8) (the default);
  * otherwise `prefetch` is the number of waves per workgroup sharing the
  * window of the register-parked kernel: (C, G, waves) in {(4,4,0), (2,2,0),

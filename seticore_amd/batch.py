@@ -187,7 +187,7 @@ FUSED_CANDIDATES = [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1), (4, 
                     (4, 8, 2), (4, 16, 2), (8, 8, 0), (8, 16, 0), (8, 4, 0)]
 FUSED32_CANDIDATES = [(4, 4, 8), (4, 8, 8), (4, 16, 8), (2, 4, 2), (2, 8, 2), (4, 8, 2)]
 STAGED_CANDIDATES = [(4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 32, 0), (4, 8, 1), (4, 16, 1),
-                     (4, 32, 1)]
+                     (4, 32, 1), (4, 256, 1)]
 TWO_PASS_CANDIDATES = [(4, 1), (4, 2), (4, 4), (8, 4), (8, 8)]
 
 _cache = {}

@@ -68,11 +68,12 @@ void default_tuning(int T, int* lanes, int* group, int* prefetch) {
   // workgroup (1.29, 2.83, 5.46, 20.1 ms against 2.25 ms for fused32_kernel at
   // 32 and 4.25, 7.83, 16.5, 33.7 ms for the two-pass search); at 256 with one
   // workgroup per offset group (20.1 ms against 21.9 for one workgroup over
-  // all four groups at 8 blocks; at 128 the split is slower, 5.58 vs 5.46).
+  // all four groups at 8 blocks; at 128 the split is slower, 5.58 vs 5.46)
+  // and all drift blocks (up to 256) in one chunk: 18.1 vs 18.8 ms at 32.
   // Below 16 time steps: 8 channels per lane, 16 drift blocks per window.
   const bool staged = T >= 32 && T <= 256;
   *lanes = (T == 16 || staged) ? 4 : 8;
-  *group = T == 16 ? 4 : T == 64 ? 16 : T == 256 ? 32 : staged ? 8 : 16;
+  *group = T == 16 ? 4 : T == 64 ? 16 : T == 256 ? 256 : staged ? 8 : 16;
   *prefetch = T == 16 ? 2 : T == 32 ? sc::kStaged32 : T == 256 ? 1 : 0;
 }
 

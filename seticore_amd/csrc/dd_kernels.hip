@@ -1701,7 +1701,8 @@ hipError_t tree_blocks_dispatch(const In* x, long stride, int T, int N, int drif
 }  // namespace
 
 bool staged_shape(int T, int C, int G, int P) {
-  const bool g_ok = G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32;
+  const bool g_ok = G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == 64 ||
+                    G == 128 || G == 256;
   if (T == 64) return C == 4 && P == 0 && g_ok;
   if (T == 128 || T == 256) return C == 4 && (P == 0 || P == 1) && g_ok;
   return T == 32 && C == 4 && P == kStaged32 && g_ok;

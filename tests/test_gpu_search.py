@@ -234,7 +234,7 @@ def test_staged_one_pass_is_bit_identical(gpu, oracle, T, G, N, max_drift):
 
 
 @pytest.mark.parametrize("T", [128, 256])
-@pytest.mark.parametrize("G", [1, 4, 32])
+@pytest.mark.parametrize("G", [1, 4, 32, 256])
 @pytest.mark.parametrize("mode", [sc.SC_STATS_HOST_EXACT, sc.SC_STATS_DEVICE])
 def test_staged_split_groups_is_bit_identical(gpu, oracle, T, G, mode):
     # Shape (4, G, 1) at 128 and 256 time steps: one workgroup per offset
@@ -275,7 +275,9 @@ def test_split_shape_only_where_compiled(gpu, T):
             dd.set_tuning(4, 8, 1)
     else:
         dd.set_tuning(4, 8, 1)
-    for G in (3, 64):
+    if T in (128, 256):
+        dd.set_tuning(4, 256, 1)  # every drift block in one chunk
+    for G in (3, 512) if T >= 128 else (3, 64):
         with pytest.raises(sc.SeticoreError):
             dd.set_tuning(4, G, 1)
 
