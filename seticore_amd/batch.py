@@ -123,7 +123,8 @@ def multibeam_instances(multibeam, num_timesteps, coarse_channel_size, foff, tsa
 
 
 class Batch:
-    """sc_batch: plan once (graph capture), run many times.
+    """sc_batch: plan once (graph capture; a gated batch: its launch plan), run
+    many times.
 
     With `proposal` (a ProposalNet), every instance is searched as
     Dedopplerer.search_proposed does, gated at `logit_threshold`; the network
@@ -131,8 +132,8 @@ class Batch:
 
     def __init__(self, instances, tuning=(), proposal=None, logit_threshold=0.0):
         L = _lib()
-        # The graph reads the instances' memory (and the network's) through
-        # raw pointers: keep them alive as long as the batch.
+        # A run reads the instances' memory (and the network's) through raw
+        # pointers: keep them alive as long as the batch.
         self._sources = [getattr(i, "_source", None) for i in instances]
         self._proposal = proposal
         self._inst = (ScInstance * len(instances))(*instances)

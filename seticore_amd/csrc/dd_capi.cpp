@@ -1499,7 +1499,13 @@ sc_status sc_batch_run(sc_batch* b, void* stream_ptr) {
   hipStream_t stream = (hipStream_t)stream_ptr;
   if (b->pn) {
     const sc_status s = enqueue_gated(b, stream);
-    if (s != SC_OK) return s;
+    if (s != SC_OK) {
+      // Let what was enqueued finish before the caller sees the error.
+      (void)hipStreamSynchronize(b->capture_stream);
+      (void)hipStreamSynchronize(b->front_stream);
+      (void)hipStreamSynchronize(b->tail_stream);
+      return s;
+    }
   } else {
     SC_HIP(hipGraphLaunch(b->exec, stream), "batch graph launch");
   }
