@@ -139,9 +139,9 @@ def _fields(hits):
 
 
 def test_gated_batch_equals_per_call_gated_search(gpu):
-    # The gated batch (one graph; each coarse channel's statistics, network
-    # and tile list on a chain of their own beside the previous channel's
-    # tree) against the per-call gated search of each coarse channel: the
+    # The gated batch (three streams: trees; each coarse channel's
+    # statistics, network and tile list beside the previous channel's tree;
+    # merges) against the per-call gated search of each coarse channel: the
     # same tiles searched, the same blocks flagged, every hit field equal -
     # FP32 and BF16 storage mixed in one batch, two runs in a row. Below
     # every logit it is the plain batch, hit for hit.
@@ -193,6 +193,48 @@ def test_gated_batch_equals_per_call_gated_search(gpu):
         assert _fields(everything.hits(i)) == _fields(plain.hits(i)), i
     with pytest.raises(sc.SeticoreError):
         plain.flagged(0)
+
+
+def test_gated_batch_of_mixed_widths(gpu):
+    # Two widths interleaved (four search handles, each instance waiting for
+    # the tail of the previous instance on its own handle): every instance
+    # equals its per-call gated search, hits and flagged blocks.
+    import torch
+    from seticore_amd import batch as B
+    widths = [1 << 17, 1 << 16, 1 << 17, 1 << 16, 1 << 16, 1 << 17, 1 << 17]
+    net = sc.ProposalNet(max_batch=max(widths) // 4096, seed=9)
+    xs = [_coarse_channel(torch, N, 60 + i) for i, N in enumerate(widths)]
+    logits = []
+    for x, N in zip(xs, widths):
+        dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
+        dd.set_stats_mode(sc.SC_STATS_DEVICE)
+        meta = sc.FilterbankMetadata(coarse_channel_size=N, foff=FOFF)
+        dd.search(sc.FilterbankBuffer(T, N, data=x), meta, -1, 0, 10.0, 0.0, 10.0, [])
+        st = dd.state()
+        stats = torch.tensor([st["median"], st["std"]], dtype=torch.float32, device="cuda")
+        logits.append(net.forward(net.pool(x, stats)).cpu())
+    thr = float(torch.cat(logits).median())  # about half of the blocks flagged
+    per_call, inst = [], []
+    for i, (x, N) in enumerate(zip(xs, widths)):
+        meta = sc.FilterbankMetadata(coarse_channel_size=N, foff=FOFF)
+        dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
+        dd.set_stats_mode(sc.SC_STATS_DEVICE)
+        out = []
+        flagged = dd.search_proposed(sc.FilterbankBuffer(T, N, data=x), net, meta, -1, i, 10.0,
+                                     0.0, 10.0, thr, out)
+        per_call.append((flagged, out))
+        inst.append(B.instance(x, T, FOFF, TSAMP, 10.0, 0.0, 10.0, coarse_channel=i,
+                               metadata=meta))
+    gb = B.Batch(inst, proposal=net, logit_threshold=thr)
+    assert gb.info()["shapes"] == 2
+    gb.run()
+    for i, (flagged, out) in enumerate(per_call):
+        assert gb.flagged(i)[1] == flagged, i
+        assert _fields(gb.hits(i)) == _fields(out), i
+    total = sum(f for f, _ in per_call)
+    print("flagged %d of %d blocks" % (total, sum(widths) // 4096))
+    assert 0 < total < sum(widths) // 4096
+    assert any(out for _, out in per_call)
 
 
 def test_trained_network_beats_the_brightest_pixel_heuristic(gpu):
