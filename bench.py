@@ -96,13 +96,15 @@ def cpu_baseline(data_cc, threads):
 
 # Committed rocprofv3 counter summaries (profiles/): HBM bytes per launch of
 # the dominant kernel, by (time steps, max drift) of the measured workload.
-TRAFFIC_FILES = {(16, 10.0): "fused_tree_cfg2_traffic.json",
-                 (256, 5.0): "staged_t256_traffic.json"}
+TRAFFIC_FILES = {("fused_tree_top_set_kernel", 16, 10.0): "fused_tree_set_cfg2_traffic.json",
+                 ("fused_tree_top_kernel", 16, 10.0): "fused_tree_cfg2_traffic.json",
+                 ("staged_tree_kernel", 256, 5.0): "staged_t256_traffic.json"}
 
 
-def load_traffic(T_r, max_drift):
-    """HBM bytes per launch of the dominant kernel for this workload, or None."""
-    name = TRAFFIC_FILES.get((T_r, max_drift))
+def load_traffic(kernel, T_r, max_drift):
+    """HBM bytes per launch of `kernel` on this workload (committed counter
+    summaries), or None."""
+    name = TRAFFIC_FILES.get((kernel, T_r, max_drift))
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
         return None
@@ -256,6 +258,7 @@ def main():
         return time.perf_counter() - t0, out
 
     loop_elapsed, loop_hits = timed(loop_step)
+    set_ms, set_launches, set_channels = 0.0, 0, 0
     if args.loop:
         elapsed, hits = loop_elapsed, loop_hits
         mode = "one Dedopplerer::search call per coarse channel"
@@ -283,23 +286,40 @@ def main():
             sys.exit(1)
         mode = "one batched graph launch over the coarse channels (hits equal to per-call searches)"
         batch_hits_cc = {cc: batch.hits(cc) for cc in range(num_cc)}
+        # The timed step's path-sum kernel, measured live: one more run of the
+        # batch's launch sequence, launched directly with HIP events around
+        # each path-sum launch (at 16 time steps one launch covers up to four
+        # coarse channels: fused_tree_top_set_kernel).
+        batch.set_timing(True)
+        batch.run()
+        set_ms, set_launches, set_channels = batch.kernel_time()
+        batch.set_timing(False)
         del batch
     if args.loop:
         batch_hits_cc = {cc: [h for h in hits if h.coarse_channel == cc] for cc in range(num_cc)}
-    # Kernel duration, measured live with HIP events on the search stream
-    # around the path-sum kernel (one extra pass, launched eagerly).
+    # The per-call kernel's duration, measured live with HIP events on the
+    # search stream around the path-sum kernel (one extra pass, launched
+    # eagerly).
     dd.set_timing(True)
     loop_step()
     torch.cuda.synchronize()
     kernel_ms, launches = dd.kernel_time()
     dd.set_timing(False)
+    per_call_kernel_ms = kernel_ms / max(1, launches)
+    channels_per_launch = 1
+    kernel_name = "fused_tree_top_kernel" if T <= 16 else "staged_tree_kernel"
+    if not args.loop and set_launches and set_channels > set_launches:
+        kernel_ms, launches = set_ms, set_launches
+        channels_per_launch = set_channels / set_launches
+        kernel_name = "fused_tree_top_set_kernel"
 
     ms_per_step = 1000.0 * elapsed / args.steps
     total_ps = per_cc * num_cc
     value = total_ps / (elapsed / args.steps) / 1e9
 
-    # Roofline of the dominant kernel (tree + top-path tracking over all drift
-    # blocks, one coarse channel per launch).
+    # Roofline of the dominant kernel of the timed step (tree + top-path
+    # tracking over all drift blocks; at 16 time steps in the batch, one launch
+    # per run of up to four coarse channels).
     # * T_r <= 16: bound by vector ALU issue. The input is read from HBM about
     #   once per launch and the drift blocks are served from LDS and L2.
     #   achieved = algorithmic VALU lane-operations per launch (path sums x
@@ -316,17 +336,22 @@ def main():
     #   fraction is reported beside it.
     avg_ms = kernel_ms / max(1, launches)
     ops_per_ps = alg_valu_ops_per_path_sum(T)
-    alg_ops = per_cc * ops_per_ps
+    ps_per_launch = int(round(per_cc * channels_per_launch))
+    alg_ops = ps_per_launch * ops_per_ps
     achieved_tops = alg_ops / (avg_ms * 1e-3) / 1e12
     peak_tops = VALU_PEAK_OPS / 1e12
-    ref_dataflow_bytes = nblocks * T * N * 4 + T * N * 4 + 12 * N
-    traffic = load_traffic(T, MAX_DRIFT)
+    ref_dataflow_bytes = int(round((nblocks * T * N * 4 + T * N * 4 + 12 * N) *
+                                   channels_per_launch))
+    traffic = load_traffic(kernel_name, T, MAX_DRIFT)
     ref_gbs = ref_dataflow_bytes / (avg_ms * 1e-3) / 1e9
 
     rl_common = {
-        "kernel": "fused_tree_top_kernel" if T <= 16 else "staged_tree_kernel",
+        "kernel": kernel_name,
         "kernel_ms_per_launch": round(avg_ms, 4),
-        "path_sums_per_launch": per_cc,
+        "coarse_channels_per_launch": round(channels_per_launch, 3),
+        "kernel_ms_per_coarse_channel": round(avg_ms / channels_per_launch, 4),
+        "per_call_kernel_ms": round(per_call_kernel_ms, 4),
+        "path_sums_per_launch": ps_per_launch,
         "hbm_traffic_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
         "hbm_traffic_frac": (round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
                              if traffic else None),

@@ -372,9 +372,10 @@ sc_status sc_dd_search_proposed(sc_dd* dd, sc_pn* pn, const void* data, int dtyp
  * No counterpart in the reference, which searches one coarse channel per call
  * (run_dedoppler.cpp:35-51, beamforming_pipeline.cpp:145-253). A batch of
  * coarse channels / beams with uneven time-step counts and channel widths is
- * planned once (one search handle per distinct shape, each instance's launch
- * sequence captured into one hipGraph) and then run as a single graph launch.
- * Statistics are the device statistics (SC_STATS_DEVICE). */
+ * planned once (search handles per distinct shape, each instance's launch
+ * sequence captured into one hipGraph; runs of up to 4 consecutive instances
+ * of one 16-step shape share one launch of the tree kernel) and then run as a
+ * single graph launch. Statistics are the device statistics (SC_STATS_DEVICE). */
 
 typedef struct {
   const void* data;          /* device [rounded_timesteps][row_stride] samples */
@@ -425,6 +426,17 @@ sc_status sc_batch_create_proposed(const sc_instance* instances, int count,
 /* Gated batch, last run: channel tiles searched and blocks flagged for one instance. */
 sc_status sc_batch_flagged_blocks(const sc_batch* b, int instance, int* tiles_searched,
                                   int* flagged_blocks);
+/* Kernel timing for measurement (plain batches; no counterpart in the
+ * reference): while enabled, sc_batch_run launches the batch's sequence
+ * directly instead of replaying its graph, with HIP events around each
+ * path-sum launch, and accumulates their elapsed time; enabling or disabling
+ * resets the totals. Runs of consecutive instances of one 16-step shape share
+ * a launch (up to 4 coarse channels), so a launch may cover several. */
+sc_status sc_batch_set_timing(sc_batch* b, int enable);
+/* Totals since sc_batch_set_timing: kernel milliseconds, path-sum launches,
+ * coarse channels those launches covered (any output may be NULL). */
+sc_status sc_batch_kernel_time(const sc_batch* b, double* total_ms, long* launches,
+                               long* channels);
 /* Number of distinct shapes and of graph nodes. */
 sc_status sc_batch_info(const sc_batch* b, int* num_shapes, int* graph_nodes);
 sc_status sc_batch_destroy(sc_batch* b);

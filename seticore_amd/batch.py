@@ -53,6 +53,9 @@ _SIGS = {
     "sc_batch_flagged_blocks": [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                 ctypes.POINTER(ctypes.c_int)],
     "sc_batch_run": [ctypes.c_void_p, ctypes.c_void_p],
+    "sc_batch_set_timing": [ctypes.c_void_p, ctypes.c_int],
+    "sc_batch_kernel_time": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)],
     "sc_batch_copy_hits": [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ScHit), ctypes.c_size_t,
                            ctypes.POINTER(ctypes.c_size_t)],
     "sc_batch_info": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
@@ -174,6 +177,18 @@ class Batch:
         a, b = ctypes.c_int(), ctypes.c_int()
         check(_lib().sc_batch_flagged_blocks(self._h, i, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_timing(self, enable):
+        """Plain batches: time the path-sum launches of later runs (direct
+        launches instead of the graph); resets the totals."""
+        check(_lib().sc_batch_set_timing(self._h, int(bool(enable))))
+
+    def kernel_time(self):
+        """(total kernel ms, path-sum launches, coarse channels covered) since set_timing."""
+        ms, n, c = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
+        check(_lib().sc_batch_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(n),
+                                          ctypes.byref(c)))
+        return ms.value, n.value, c.value
 
     def info(self):
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -1219,6 +1219,11 @@ struct sc_batch {
   hipStream_t front_stream = nullptr;
   std::vector<hipEvent_t> gate_events;
   std::vector<long> prev_on_handle;
+  // Kernel timing (plain batches): runs launch directly, with events around
+  // each path-sum launch.
+  bool timing = false;
+  double kernel_ms = 0.0;
+  long kernel_launches = 0, kernel_channels = 0;
 };
 
 namespace {
@@ -1303,6 +1308,164 @@ sc_status enqueue_gated(sc_batch* b, hipStream_t stream) {
   return SC_OK;
 }
 
+struct BatchTimer {
+  hipEvent_t start, stop;
+  int channels;
+};
+
+// The plain batch's launch sequence, into streams M (path sums) and Q
+// (statistics, merge, copies), captured or launched directly: every
+// instance's tail hangs off its path sums on Q; an instance's path sums wait
+// for the tail of the previous instance on the same handle. Runs of
+// consecutive instances of one 16-step shape share one launch of the fused
+// kernel. With `timers`, a pair of timing events brackets each path-sum
+// launch (with its channel count). new_event creates an event the caller
+// destroys; *tail_done receives the last tail event, *err the first HIP error.
+template <typename NewEvent>
+sc_status enqueue_plain(sc_batch* b, hipStream_t M, hipStream_t Q, NewEvent&& new_event,
+                        std::vector<BatchTimer>* timers,
+                        hipEvent_t* tail_done_out, hipError_t* err) {
+  hipError_t& e = *err;
+  sc_status s = SC_OK;
+  hipEvent_t tail_done = nullptr;
+  // last_tail[h]: the tail event of the previous instance on handle h.
+  std::vector<hipEvent_t> last_tail(b->shapes.size(), nullptr);
+  // Whether instance j can join a set launch with instance i: the same
+  // shape and launch parameters, on another handle.
+  auto same_launch = [&](const sc_batch::Instance& a, const sc_batch::Instance& c) {
+    const sc_dd* da = b->shapes[a.shape];
+    const sc_dd* dc = b->shapes[c.shape];
+    return a.shape != c.shape && da->rounded == dc->rounded &&
+           da->num_channels == dc->num_channels && da->foff == dc->foff &&
+           da->tsamp == dc->tsamp && a.in.dtype == c.in.dtype &&
+           a.in.row_stride == c.in.row_stride && a.r.bmin == c.r.bmin &&
+           a.r.bmax == c.r.bmax && da->lanes == dc->lanes && da->group == dc->group &&
+           da->prefetch == dc->prefetch;
+  };
+  for (size_t i = 0; i < b->inst.size() && s == SC_OK && e == hipSuccess;) {
+    // A run of up to kMaxFusedSet consecutive instances searched by one
+    // launch of the fused kernel (T_r = 16, default shape), else one.
+    size_t n = 1;
+    {
+      const sc_dd* d0 = b->shapes[b->inst[i].shape];
+      if (d0->rounded == 16 && !d0->use_two_pass &&
+          sc::fused_set_supported(16, d0->lanes, d0->group, d0->prefetch)) {
+        while (n < (size_t)sc::kMaxFusedSet && i + n < b->inst.size()) {
+          bool ok = same_launch(b->inst[i], b->inst[i + n]);
+          for (size_t m = 1; ok && m < n; ++m)
+            ok = b->inst[i + m].shape != b->inst[i + n].shape;
+          if (!ok) break;
+          ++n;
+        }
+      }
+    }
+    for (size_t j = i; j < i + n && e == hipSuccess; ++j)
+      if (last_tail[b->inst[j].shape])
+        e = hipStreamWaitEvent(M, last_tail[b->inst[j].shape], 0);
+    if (e == hipSuccess && timers) {
+      hipEvent_t t0 = nullptr, t1 = nullptr;
+      e = hipEventCreate(&t0);
+      if (e == hipSuccess) e = hipEventCreate(&t1);
+      if (e == hipSuccess) timers->push_back({t0, t1, (int)n});
+    }
+    if (e != hipSuccess) break;
+    if (n == 1) {
+      sc_batch::Instance& bi = b->inst[i];
+      if (timers) e = hipEventRecord(timers->back().start, M);  // (bracket includes the reset)
+      if (e == hipSuccess)
+        s = enqueue_path_sums(b->shapes[bi.shape], bi.in.data, bi.in.dtype, bi.in.row_stride,
+                              bi.r, M, false, true);
+    } else {
+      // dedoppler.cu:205 per channel, then the set launch (column sums
+      // written by the workgroups that hold drift block 0).
+      sc::FusedSet set = {};
+      for (size_t j = 0; j < n && e == hipSuccess; ++j) {
+        sc_batch::Instance& bj = b->inst[i + j];
+        sc_dd* dj = b->shapes[bj.shape];
+        e = sc::launch_reset(dj->top_key, dj->num_channels, dj->stats_hist, M);
+        set.x[j] = bj.in.data;
+        set.key[j] = dj->top_key;
+        set.colsum[j] = dj->colsum;
+      }
+      const sc_batch::Instance& b0 = b->inst[i];
+      const sc_dd* d0 = b->shapes[b0.shape];
+      if (e == hipSuccess && timers) e = hipEventRecord(timers->back().start, M);
+      if (e == hipSuccess)
+        e = sc::launch_fused16_set(set, (int)n, b0.in.dtype, b0.in.row_stride, 16,
+                                   d0->num_channels, b0.r.bmin, b0.r.bmax - b0.r.bmin + 1,
+                                   d0->lanes, d0->group, d0->prefetch, true,
+                                   M);
+      if (e != hipSuccess) s = hip_fail(e, "set launch");
+    }
+    if (s == SC_OK && e == hipSuccess && timers) e = hipEventRecord(timers->back().stop, M);
+    if (s != SC_OK) break;
+    hipEvent_t sums = nullptr;
+    e = new_event(&sums);
+    if (e == hipSuccess) e = hipEventRecord(sums, M);
+    if (e == hipSuccess) e = hipStreamWaitEvent(Q, sums, 0);
+    if (e != hipSuccess) break;
+    for (size_t j = i; j < i + n && s == SC_OK && e == hipSuccess; ++j) {
+      sc_batch::Instance& bj = b->inst[j];
+      sc_dd* dj = b->shapes[bj.shape];
+      s = enqueue_device_tail(dj, bj.r, bj.in.snr_threshold, bj.in.data, bj.in.dtype,
+                              bj.in.row_stride, Q, b->h_cand + bj.cand_offset,
+                              b->h_stats + 4 * j);
+      if (s != SC_OK) break;
+      hipEvent_t tail = nullptr;
+      e = new_event(&tail);
+      if (e == hipSuccess) e = hipEventRecord(tail, Q);
+      last_tail[bj.shape] = tail;
+      tail_done = tail;
+    }
+    i += n;
+  }
+  *tail_done_out = tail_done;
+  if (e != hipSuccess && s == SC_OK) s = hip_fail(e, "batch launches");
+  return s;
+}
+
+// A plain batch run as direct launches with timing events (sc_batch_set_timing):
+// the same sequence as the captured graph, on the same two streams.
+sc_status run_plain_timed(sc_batch* b, hipStream_t stream) {
+  std::vector<hipEvent_t> events;
+  auto new_event = [&](hipEvent_t* ev) {
+    hipError_t e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    if (e == hipSuccess) events.push_back(*ev);
+    return e;
+  };
+  std::vector<BatchTimer> timers;
+  hipEvent_t start = nullptr, tail_done = nullptr;
+  hipError_t e = new_event(&start);
+  if (e == hipSuccess) e = hipEventRecord(start, stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->capture_stream, start, 0);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->tail_stream, start, 0);
+  sc_status s = e == hipSuccess ? enqueue_plain(b, b->capture_stream, b->tail_stream, new_event,
+                                                &timers, &tail_done, &e)
+                                : hip_fail(e, "batch start");
+  // The last tail follows the last path sums: waiting for it joins both streams.
+  if (s == SC_OK && tail_done) {
+    e = hipStreamWaitEvent(stream, tail_done, 0);
+    if (e != hipSuccess) s = hip_fail(e, "batch done");
+  }
+  (void)hipStreamSynchronize(b->capture_stream);
+  (void)hipStreamSynchronize(b->tail_stream);
+  (void)hipStreamSynchronize(stream);
+  for (const BatchTimer& t : timers) {
+    float ms = 0.0f;
+    if (s == SC_OK) {
+      e = hipEventElapsedTime(&ms, t.start, t.stop);
+      if (e != hipSuccess) s = hip_fail(e, "kernel timing");
+    }
+    b->kernel_ms += ms;
+    b->kernel_launches += 1;
+    b->kernel_channels += t.channels;
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
+  return s;
+}
+
 sc_status batch_create(const sc_instance* instances, int count, const sc_tuning* tuning,
                        int num_tuning, sc_pn* pn, float logit_threshold, sc_batch** out) {
   if (!out) return fail(SC_ERR_INVALID_ARGUMENT, "null output handle");
@@ -1339,7 +1502,10 @@ sc_status batch_create(const sc_instance* instances, int count, const sc_tuning*
     }
     if (shape < 0) {
       shape = (int)b->shapes.size();
-      int copies = 2;
+      // Two handles per shape, used alternately; at 16 time steps two per
+      // channel of a set launch (runs of consecutive instances of the shape
+      // share one launch of the fused kernel, below).
+      int copies = T == 16 ? 2 * sc::kMaxFusedSet : 2;
       for (int k = 0; k < num_tuning; ++k)
         if (tuning[k].rounded_timesteps == T && tuning[k].num_channels == in.num_channels &&
             T > 16 && tuning[k].pass_group != 0)
@@ -1425,31 +1591,9 @@ sc_status batch_create(const sc_instance* instances, int count, const sc_tuning*
   if (s == SC_OK && !pn) {
     hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) s = hip_fail(e, "begin capture");
-    // last_tail[h]: the tail event of the previous instance on handle h.
-    std::vector<hipEvent_t> last_tail(b->shapes.size(), nullptr);
     hipEvent_t tail_done = nullptr;
-    for (size_t i = 0; i < b->inst.size() && s == SC_OK && e == hipSuccess; ++i) {
-      sc_batch::Instance& bi = b->inst[i];
-      sc_dd* dd = b->shapes[bi.shape];
-      if (last_tail[bi.shape]) e = hipStreamWaitEvent(b->capture_stream, last_tail[bi.shape], 0);
-      if (e != hipSuccess) break;
-      s = enqueue_path_sums(dd, bi.in.data, bi.in.dtype, bi.in.row_stride, bi.r,
-                            b->capture_stream, false, true);
-      if (s != SC_OK) break;
-      hipEvent_t sums = nullptr, tail = nullptr;
-      e = new_event(&sums);
-      if (e == hipSuccess) e = new_event(&tail);
-      if (e == hipSuccess) e = hipEventRecord(sums, b->capture_stream);
-      if (e == hipSuccess) e = hipStreamWaitEvent(b->tail_stream, sums, 0);
-      if (e != hipSuccess) break;
-      s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, bi.in.data, bi.in.dtype,
-                              bi.in.row_stride, b->tail_stream, b->h_cand + bi.cand_offset,
-                              b->h_stats + 4 * i);
-      if (s != SC_OK) break;
-      e = hipEventRecord(tail, b->tail_stream);
-      last_tail[bi.shape] = tail;
-      tail_done = tail;
-    }
+    if (e == hipSuccess)
+      s = enqueue_plain(b, b->capture_stream, b->tail_stream, new_event, nullptr, &tail_done, &e);
     // Join the tail chain back into the capturing stream.
     if (e == hipSuccess && s == SC_OK && tail_done)
       e = hipStreamWaitEvent(b->capture_stream, tail_done, 0);
@@ -1497,7 +1641,10 @@ sc_status sc_batch_create_proposed(const sc_instance* instances, int count,
 sc_status sc_batch_run(sc_batch* b, void* stream_ptr) {
   if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
   hipStream_t stream = (hipStream_t)stream_ptr;
-  if (b->pn) {
+  if (!b->pn && b->timing) {
+    const sc_status s = run_plain_timed(b, stream);
+    if (s != SC_OK) return s;
+  } else if (b->pn) {
     const sc_status s = enqueue_gated(b, stream);
     if (s != SC_OK) {
       // Let what was enqueued finish before the caller sees the error.
@@ -1542,6 +1689,25 @@ sc_status sc_batch_flagged_blocks(const sc_batch* b, int instance, int* tiles_se
     return fail(SC_ERR_INVALID_ARGUMENT, "instance out of range");
   if (tiles_searched) *tiles_searched = b->h_gate[2 * instance];
   if (flagged_blocks) *flagged_blocks = b->h_gate[2 * instance + 1];
+  return SC_OK;
+}
+
+sc_status sc_batch_set_timing(sc_batch* b, int enable) {
+  if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
+  if (b->pn) return fail(SC_ERR_UNSUPPORTED, "kernel timing is for plain batches");
+  b->timing = enable != 0;
+  b->kernel_ms = 0.0;
+  b->kernel_launches = 0;
+  b->kernel_channels = 0;
+  return SC_OK;
+}
+
+sc_status sc_batch_kernel_time(const sc_batch* b, double* total_ms, long* launches,
+                               long* channels) {
+  if (!b) return fail(SC_ERR_INVALID_ARGUMENT, "null batch");
+  if (total_ms) *total_ms = b->kernel_ms;
+  if (launches) *launches = b->kernel_launches;
+  if (channels) *channels = b->kernel_channels;
   return SC_OK;
 }
 

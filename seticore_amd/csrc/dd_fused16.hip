@@ -108,6 +108,107 @@ __global__ void __launch_bounds__(kWave)
   }
 }
 
+// Several coarse channels of one shape in one launch (a batch's consecutive
+// instances): items are instance-major, then tile-major, dealt over the XCDs
+// as above. Longer chunks with the same number of items: at 16 x 2^20 x 4 the
+// launcher makes two chunks per tile instead of about eight. The body below
+// the item decode is fused_tree_top_kernel's, repeated as text: written as one
+// shared inline function, the kernels compiled to ten more registers and the
+// single-channel one ran 2-3 % slower.
+template <int T, int C, int G, int MODE, typename In, bool BK>
+__global__ void __launch_bounds__(kWave)
+    fused_tree_top_set_kernel(FusedSet set, long stride, int N, int bmin, int nblocks,
+                              int ngroups, int chunks, int groups_per_chunk,
+                              int items_per_channel, int total_items) {
+  using Sh = TileShape<T, C>;
+  using Wn = Window<T, C, G>;
+  __shared__ float win_storage[Wn::kFloats];
+  lds_float* win = (lds_float*)win_storage;
+
+  const int per_xcd = gridDim.x / 8;
+  const int item_all = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (item_all >= total_items) return;
+  const int inst = item_all / items_per_channel;
+  const int item = item_all - inst * items_per_channel;
+  const In* __restrict__ x = (const In*)set.x[inst];
+  unsigned long long* __restrict__ top_key = set.key[inst];
+  float* __restrict__ colsum = set.colsum[inst];
+  const int tile = item / chunks;
+  const int chunk = item % chunks;
+  const int c_w = tile * Sh::kOut;
+  const int g_begin = chunk * groups_per_chunk;
+  const int g_end = min(ngroups, g_begin + groups_per_chunk);
+  const int lane = threadIdx.x;
+
+  float pf[T][Wn::kLoads];
+  fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + g_begin * G, 0);
+  const int b_first = bmin + g_begin * G;
+  const int b_last = min(bmin + g_end * G, bmin + nblocks) - 1;
+  if (colsum && b_first <= 0 && 0 <= b_last && lane < Sh::kValidLanes) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int c = c_w + lane * C + j;
+      if (c < N) {
+        float s = 0.0f;
+#pragma unroll
+        for (int t = 0; t < T; ++t) s += to_f32<In>(x[(long)t * stride + c]);
+        colsum[c] = s;
+      }
+    }
+  }
+  store_window<T, C, G>(win, pf);
+  __syncthreads();
+
+  Best<C, BK> best;
+  best.init();
+
+  for (int grp = g_begin; grp < g_end; ++grp) {
+    // MODE 0: every work item is one drift group (the launcher makes
+    // groups_per_chunk 1), so there is no next window to fetch.
+    const bool more = MODE != 0 && grp + 1 < g_end;
+    if (MODE == 1 && more)
+      fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + (grp + 1) * G, 0);
+    const int b0 = bmin + grp * G;
+    const int ng = min(G, nblocks - grp * G);
+    lds_float* rp[T];
+    row_bases<T, C, G>(win, rp);
+    for (int g = 0; g < ng; g += C) {
+      fused_block<T, C, G, BK, 0>(rp, g, ng, b0, c_w, N, grp * G, best);
+#pragma unroll
+      for (int t = 1; t < T; ++t) rp[t] += t;  // next h step: rows shift by t columns
+    }
+    if (more) {
+      if (MODE == 2) fetch_window<T, C, G, In>(pf, x, stride, N, c_w, bmin + (grp + 1) * G, 0);
+      __syncthreads();
+      store_window<T, C, G>(win, pf);
+      __syncthreads();
+    }
+  }
+
+  if (lane < Sh::kValidLanes) {
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int c = c_w + lane * C + j;
+      const unsigned long long k = best.merge_key(j);
+      if (c < N && k != 0) atomicMax(&top_key[c], k);
+    }
+  }
+}
+
+
+// Chunks per tile so that about `target` items exist (mode 0: one group each).
+template <int MODE>
+long fused_chunks(long tiles, long ngroups) {
+  long chunks = ngroups;
+  if (MODE != 0) {
+    const long target = MODE == 1 ? 8192 : 32768;
+    chunks = 1;
+    while (tiles * chunks < target && chunks < ngroups) ++chunks;
+  }
+  const long per_chunk = (ngroups + chunks - 1) / chunks;
+  return (ngroups + per_chunk - 1) / per_chunk;
+}
+
 template <int T, int C, int G, int MODE, typename In>
 hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nblocks,
                               top_key_t* top_key, float* colsum, const int* tile_list,
@@ -119,14 +220,8 @@ hipError_t launch_fused_typed(const In* x, long stride, int N, int bmin, int nbl
   // filled several times over (256 CUs x a few resident workgroups). Measured
   // at 16 x 2^20, 132 blocks, mode 2: 4096 / 8192 / 16384 / 32768 / 65536
   // items -> 0.64 / 0.56 / 0.54 / 0.49 / 0.49 ms.
-  long chunks = ngroups;
-  if (MODE != 0) {
-    const long target = MODE == 1 ? 8192 : 32768;
-    chunks = 1;
-    while (ntiles * chunks < target && chunks < ngroups) ++chunks;
-  }
+  const long chunks = fused_chunks<MODE>(ntiles, ngroups);
   const long per_chunk = (ngroups + chunks - 1) / chunks;
-  chunks = (ngroups + per_chunk - 1) / per_chunk;
   const long total = ntiles * chunks;
   if (total > (1L << 30)) return hipErrorInvalidValue;
   const int grid = (int)((total + 7) / 8 * 8);
@@ -156,7 +251,50 @@ hipError_t fused_dispatch(const In* x, long stride, int T, int N, int bmin, int 
   return hipErrorInvalidValue;
 }
 
+template <typename In>
+hipError_t launch_set_typed(const FusedSet& set, int n, long stride, int N, int bmin,
+                            int nblocks, bool block_keys, hipStream_t stream) {
+  // The default 16-time-step shape (C = 4, G = 4, mode 2).
+  constexpr int T = 16, C = 4, G = 4, MODE = 2;
+  using Sh = TileShape<T, C>;
+  const long ntiles = ((long)N + Sh::kOut - 1) / Sh::kOut;
+  const long ngroups = ((long)nblocks + G - 1) / G;
+  const long chunks = fused_chunks<MODE>(ntiles * n, ngroups);
+  const long per_chunk = (ngroups + chunks - 1) / chunks;
+  const long per_channel = ntiles * chunks;
+  const long total = per_channel * n;
+  if (total > (1L << 30)) return hipErrorInvalidValue;
+  const int grid = (int)((total + 7) / 8 * 8);
+  if (block_keys)
+    hipLaunchKernelGGL((fused_tree_top_set_kernel<T, C, G, MODE, In, true>), dim3(grid),
+                       dim3(kWave), 0, stream, set, stride, N, bmin, nblocks, (int)ngroups,
+                       (int)chunks, (int)per_chunk, (int)per_channel, (int)total);
+  else
+    hipLaunchKernelGGL((fused_tree_top_set_kernel<T, C, G, MODE, In, false>), dim3(grid),
+                       dim3(kWave), 0, stream, set, stride, N, bmin, nblocks, (int)ngroups,
+                       (int)chunks, (int)per_chunk, (int)per_channel, (int)total);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+bool fused_set_supported(int T, int C, int G, int P) {
+  return T == 16 && C == 4 && G == 4 && P == 2;
+}
+
+hipError_t launch_fused16_set(const FusedSet& set, int n, int dtype, long stride, int T, int N,
+                              int bmin, int nblocks, int C, int G, int P, bool block_keys,
+                              hipStream_t stream) {
+  if (n < 1 || n > kMaxFusedSet || !fused_set_supported(T, C, G, P)) return hipErrorInvalidValue;
+  switch (dtype) {
+    case 0: return launch_set_typed<float>(set, n, stride, N, bmin, nblocks, block_keys, stream);
+    case 1: return launch_set_typed<__half>(set, n, stride, N, bmin, nblocks, block_keys, stream);
+    case 2:
+      return launch_set_typed<__hip_bfloat16>(set, n, stride, N, bmin, nblocks, block_keys,
+                                              stream);
+  }
+  return hipErrorInvalidValue;
+}
 
 hipError_t launch_fused16(const void* x, int dtype, long stride, int T, int N, int bmin,
                           int nblocks, int C, int G, int P, top_key_t* top_key, float* colsum,

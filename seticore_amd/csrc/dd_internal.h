@@ -60,6 +60,21 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
                                  const int* tile_list = nullptr,
                                  const int* tile_count = nullptr, bool block_keys = false);
 int fused_tile_width(int T, int C);
+// Up to kMaxFusedSet coarse channels of one shape (same T, N, row stride,
+// sample type and drift range) searched by one launch of the T_r = 16 kernel.
+constexpr int kMaxFusedSet = 4;
+struct FusedSet {
+  const void* x[kMaxFusedSet];
+  top_key_t* key[kMaxFusedSet];
+  float* colsum[kMaxFusedSet];  // may be null (column sums not written)
+};
+// Whether launch_fused16_set has a kernel for (T, C, G, P) (the default
+// 16-time-step shape).
+bool fused_set_supported(int T, int C, int G, int P);
+hipError_t launch_fused16_set(const FusedSet& set, int n, int dtype, long stride, int T, int N,
+                              int bmin, int nblocks, int C, int G, int P, bool block_keys,
+                              hipStream_t stream);
+
 // The T_r <= 16 kernel of launch_fused_tree_top (dd_fused16.hip), same arguments.
 hipError_t launch_fused16(const void* x, int dtype, long stride, int T, int N, int bmin,
                           int nblocks, int C, int G, int P, top_key_t* top_key, float* colsum,

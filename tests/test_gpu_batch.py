@@ -145,3 +145,48 @@ def test_multibeam_regions_in_place(gpu, oracle):
         same += summary["same"]
         total += len(got)
     assert total > 0 and same >= total - 2
+
+
+def test_runs_of_one_shape_share_a_launch(gpu, oracle):
+    # Consecutive instances of one 16-step shape are searched by one launch of
+    # the fused kernel over up to four coarse channels (runs of 4, 4 and 1
+    # here; the instance of another shape in between ends a run). Each equals
+    # its own per-call search, and the first its oracle search (statistics
+    # differences only between the two thresholds).
+    import torch
+    N, T = 1 << 17, 16
+    rng = np.random.default_rng(21)
+    data, inst = [], []
+    for i in range(10):
+        n = N if i != 4 else N // 2
+        x = torch.empty((T, n), dtype=torch.float32, device="cuda")
+        tones = [(rng.uniform(0.1 * n, 0.9 * n), rng.uniform(-1.5, 1.5), rng.uniform(3, 9))
+                 for _ in range(8)]
+        sc.gen_waterfall(x, T, seed=300 + i, tones=tones)
+        data.append(x)
+        inst.append(B.instance(x, T, FOFF, TSAMP, 5.0, 0.0, 8.0, coarse_channel=i))
+    want = _reference_hits(data, [B.instance(x, T, FOFF, TSAMP, 5.0, 0.0, 10.0,
+                                              coarse_channel=i) for i, x in enumerate(data)])
+    b = B.Batch([B.instance(x, T, FOFF, TSAMP, 5.0, 0.0, 10.0, coarse_channel=i)
+                 for i, x in enumerate(data)])
+    for _ in range(2):
+        b.run()
+        assert _batch_hits(b) == want
+    assert sum(len(w) for w in want) > 0
+    # Kernel timing: the same sequence launched directly, four path-sum
+    # launches (4, 1, 4 and 1 coarse channels), the same hits.
+    b.set_timing(True)
+    b.run()
+    assert _batch_hits(b) == want
+    ms, launches, channels = b.kernel_time()
+    assert (launches, channels) == (4, 10) and ms > 0
+    b.set_timing(False)
+    b.run()
+    assert _batch_hits(b) == want and b.kernel_time() == (0.0, 0, 0)
+    # Against the oracle, at SNR 8 (more hits).
+    b8 = B.Batch(inst)
+    b8.run()
+    x0 = data[0].cpu().numpy()
+    ref = oracle.search(x0, T, FOFF, TSAMP, 5.0, 0.0, 8.0, num_threads=8)
+    summary = device_hits_match_oracle(b8.hits(0), ref, x0, 8.0, T - 1)
+    assert summary["same"] > 0
