@@ -288,7 +288,7 @@ def main():
         batch_hits_cc = {cc: batch.hits(cc) for cc in range(num_cc)}
         # The timed step's path-sum kernel, measured live: one more run of the
         # batch's launch sequence, launched directly with HIP events around
-        # each path-sum launch (at 16 time steps one launch covers up to four
+        # each path-sum launch (at 16 time steps one launch covers up to eight
         # coarse channels: fused_tree_top_set_kernel).
         batch.set_timing(True)
         batch.run()
@@ -319,7 +319,7 @@ def main():
 
     # Roofline of the dominant kernel of the timed step (tree + top-path
     # tracking over all drift blocks; at 16 time steps in the batch, one launch
-    # per run of up to four coarse channels).
+    # per run of up to eight coarse channels).
     # * T_r <= 16: bound by vector ALU issue. The input is read from HBM about
     #   once per launch and the drift blocks are served from LDS and L2.
     #   achieved = algorithmic VALU lane-operations per launch (path sums x

@@ -373,7 +373,7 @@ sc_status sc_dd_search_proposed(sc_dd* dd, sc_pn* pn, const void* data, int dtyp
  * (run_dedoppler.cpp:35-51, beamforming_pipeline.cpp:145-253). A batch of
  * coarse channels / beams with uneven time-step counts and channel widths is
  * planned once (search handles per distinct shape, each instance's launch
- * sequence captured into one hipGraph; runs of up to 4 consecutive instances
+ * sequence captured into one hipGraph; runs of up to 8 consecutive instances
  * of one 16-step shape share one launch of the tree kernel) and then run as a
  * single graph launch. Statistics are the device statistics (SC_STATS_DEVICE). */
 
@@ -431,7 +431,7 @@ sc_status sc_batch_flagged_blocks(const sc_batch* b, int instance, int* tiles_se
  * directly instead of replaying its graph, with HIP events around each
  * path-sum launch, and accumulates their elapsed time; enabling or disabling
  * resets the totals. Runs of consecutive instances of one 16-step shape share
- * a launch (up to 4 coarse channels), so a launch may cover several. */
+ * a launch (up to 8 coarse channels), so a launch may cover several. */
 sc_status sc_batch_set_timing(sc_batch* b, int enable);
 /* Totals since sc_batch_set_timing: kernel milliseconds, path-sum launches,
  * coarse channels those launches covered (any output may be NULL). */

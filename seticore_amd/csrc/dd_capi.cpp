@@ -1502,10 +1502,10 @@ sc_status batch_create(const sc_instance* instances, int count, const sc_tuning*
     }
     if (shape < 0) {
       shape = (int)b->shapes.size();
-      // Two handles per shape, used alternately; at 16 time steps two per
-      // channel of a set launch (runs of consecutive instances of the shape
-      // share one launch of the fused kernel, below).
-      int copies = T == 16 ? 2 * sc::kMaxFusedSet : 2;
+      // Two handles per shape, used alternately; in a plain batch at 16 time
+      // steps two per channel of a set launch (runs of consecutive instances
+      // of the shape share one launch of the fused kernel, enqueue_plain).
+      int copies = T == 16 && !pn ? 2 * sc::kMaxFusedSet : 2;
       for (int k = 0; k < num_tuning; ++k)
         if (tuning[k].rounded_timesteps == T && tuning[k].num_channels == in.num_channels &&
             T > 16 && tuning[k].pass_group != 0)

@@ -110,8 +110,8 @@ __global__ void __launch_bounds__(kWave)
 
 // Several coarse channels of one shape in one launch (a batch's consecutive
 // instances): items are instance-major, then tile-major, dealt over the XCDs
-// as above. Longer chunks with the same number of items: at 16 x 2^20 x 4 the
-// launcher makes two chunks per tile instead of about eight. The body below
+// as above. Longer chunks with the same number of items: at 16 x 2^20 x 8 the
+// launcher makes one chunk per tile instead of about eight. The body below
 // the item decode is fused_tree_top_kernel's, repeated as text: written as one
 // shared inline function, the kernels compiled to ten more registers and the
 // single-channel one ran 2-3 % slower.

@@ -62,7 +62,7 @@ hipError_t launch_fused_tree_top(const void* x, int dtype, long stride, int T, i
 int fused_tile_width(int T, int C);
 // Up to kMaxFusedSet coarse channels of one shape (same T, N, row stride,
 // sample type and drift range) searched by one launch of the T_r = 16 kernel.
-constexpr int kMaxFusedSet = 4;
+constexpr int kMaxFusedSet = 8;
 struct FusedSet {
   const void* x[kMaxFusedSet];
   top_key_t* key[kMaxFusedSet];
