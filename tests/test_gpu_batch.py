@@ -190,3 +190,35 @@ def test_runs_of_one_shape_share_a_launch(gpu, oracle):
     ref = oracle.search(x0, T, FOFF, TSAMP, 5.0, 0.0, 8.0, num_threads=8)
     summary = device_hits_match_oracle(b8.hits(0), ref, x0, 8.0, T - 1)
     assert summary["same"] > 0
+
+
+@pytest.mark.parametrize("dtype", [sc.SC_F16, sc.SC_BF16])
+def test_half_width_runs_share_a_launch(gpu, dtype):
+    # Five normalised coarse channels of one 16-step shape in FP16 or BF16
+    # storage: one set launch, each channel equal to its per-call search on
+    # the same samples.
+    import torch
+    N, T = 1 << 16, 16
+    rng = np.random.default_rng(40 + dtype)
+    data, inst, want = [], [], []
+    for i in range(5):
+        x = torch.empty((T, N), dtype=torch.float32, device="cuda")
+        tones = [(rng.uniform(0.1 * N, 0.9 * N), rng.uniform(-1, 1), rng.uniform(4, 9))
+                 for _ in range(6)]
+        sc.gen_waterfall(x, T, seed=500 + i, tones=tones)
+        xh = sc.normalize(x, dtype)[0]
+        data.append(xh)
+        inst.append(B.instance(xh, T, FOFF, TSAMP, 5.0, 0.0, 8.0, coarse_channel=i, dtype=dtype))
+        dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
+        dd.set_stats_mode(sc.SC_STATS_DEVICE)
+        hits = []
+        dd.search(sc.FilterbankBuffer(T, N, data=xh, dtype=dtype),
+                  sc.FilterbankMetadata(coarse_channel_size=N, foff=FOFF), -1, i, 5.0, 0.0,
+                  8.0, hits)
+        want.append([h.key() + (h.snr, h.power, h.num_timesteps) for h in hits])
+    b = B.Batch(inst)
+    b.set_timing(True)
+    b.run()
+    assert b.kernel_time()[1:] == (1, 5)  # one launch over the five channels
+    assert _batch_hits(b) == want
+    assert sum(len(w) for w in want) > 0
