@@ -614,6 +614,23 @@ def pn_num_params():
     return lib().sc_pn_num_params()
 
 
+def _check_tiles(tiles, blocks=None):
+    """Pooled tiles as the network's kernels read them: a contiguous CUDA
+    bfloat16 tensor [blocks][16][1024]."""
+    torch = _torch()
+    if (not tiles.is_cuda or tiles.dtype != torch.bfloat16 or not tiles.is_contiguous() or
+            tuple(tiles.shape[1:]) != (TILE_ROWS, TILE_COLUMNS) or
+            (blocks is not None and tiles.shape[0] != blocks)):
+        raise ValueError("tiles must be a contiguous CUDA bfloat16 tensor [%s][%d][%d]" %
+                         ("blocks" if blocks is None else blocks, TILE_ROWS, TILE_COLUMNS))
+
+
+def _check_labels(labels, blocks):
+    torch = _torch()
+    if not labels.is_cuda or labels.dtype != torch.float32 or labels.numel() < blocks:
+        raise ValueError("labels must be a CUDA float32 tensor of one 0/1 value per block")
+
+
 class ProposalNet:
     """The block proposal network: BF16 weights/activations, FP32 master
     weights and Adam state on the device. Tiles are BF16 CUDA tensors
@@ -655,20 +672,32 @@ class ProposalNet:
         return n.value
 
     @staticmethod
-    def pool(data, stats, dtype=SC_F32, out=None, stream=None):
+    def pool(data, stats, dtype=None, out=None, stream=None):
         """BF16 tiles [N/4096][16][1024] of a CUDA [16][N] coarse channel;
-        stats = CUDA float32 tensor {median, std} of its column sums."""
+        stats = CUDA float32 tensor {median, std} of its column sums. The
+        sample type is the tensor's (dtype, if given, must agree)."""
         torch = _torch()
+        codes = {torch.float32: SC_F32, torch.float16: SC_F16, torch.bfloat16: SC_BF16}
+        if data.dim() != 2 or not data.is_cuda or data.stride(1) != 1 or data.dtype not in codes:
+            raise ValueError("pool data must be a CUDA [16][N] float32/float16/bfloat16 tensor "
+                             "with unit channel stride")
+        if dtype is not None and dtype != codes[data.dtype]:
+            raise ValueError("pool data dtype %s does not match sample type %r" % (data.dtype,
+                                                                                   dtype))
+        if not stats.is_cuda or stats.dtype != torch.float32 or stats.numel() < 2:
+            raise ValueError("stats must be a CUDA float32 tensor {median, std}")
         T, N = data.shape
         if out is None:
             out = torch.empty((N // BLOCK_CHANNELS, TILE_ROWS, TILE_COLUMNS), dtype=torch.bfloat16,
                               device=data.device)
-        check(lib().sc_pn_pool(data.data_ptr(), int(dtype), T, N, data.stride(0),
+        _check_tiles(out, N // BLOCK_CHANNELS)
+        check(lib().sc_pn_pool(data.data_ptr(), int(codes[data.dtype]), T, N, data.stride(0),
                                stats.data_ptr(), out.data_ptr(), _stream_ptr(stream)))
         return out
 
     def forward(self, tiles, stream=None):
         torch = _torch()
+        _check_tiles(tiles)
         B = tiles.shape[0]
         logits = torch.empty(B, dtype=torch.float32, device=tiles.device)
         check(lib().sc_pn_forward(self._h, tiles.data_ptr(), B, logits.data_ptr(),
@@ -678,6 +707,8 @@ class ProposalNet:
     def gradient(self, tiles, labels, pos_weight=1.0, stream=None):
         """(grad tensor [num_params], loss tensor [1])."""
         torch = _torch()
+        _check_tiles(tiles)
+        _check_labels(labels, tiles.shape[0])
         grad = torch.empty(pn_num_params(), dtype=torch.float32, device=tiles.device)
         loss = torch.empty(1, dtype=torch.float32, device=tiles.device)
         check(lib().sc_pn_gradient(self._h, tiles.data_ptr(), labels.data_ptr(), tiles.shape[0],
@@ -686,6 +717,8 @@ class ProposalNet:
         return grad, loss
 
     def train_step(self, tiles, labels, lr, pos_weight=1.0, loss=None, stream=None):
+        _check_tiles(tiles)
+        _check_labels(labels, tiles.shape[0])
         check(lib().sc_pn_train_step(self._h, tiles.data_ptr(), labels.data_ptr(),
                                      tiles.shape[0], float(lr), float(pos_weight),
                                      loss.data_ptr() if loss is not None else None,

@@ -1186,12 +1186,14 @@ extern "C" sc_status sc_condition_coarse_channel(float* data, int num_timesteps,
 // chain, so it can overlap the next instance's path sums. Instances of one
 // shape (rounded time steps, channels, foff, tsamp) share search handles:
 // two, used alternately (measured +10 % at 16 x 2^20 and +1.6 % on the uneven
-// 16..256 batch), or one for a shape tuned to the two-pass search (whose
-// scratch is large); an instance's path sums wait for the tail of the
-// previous instance on the same handle.
+// 16..256 batch), 2 x kMaxFusedSet at 16 time steps (runs of consecutive
+// instances share one launch of the fused kernel, each on its own handle), or
+// one for a shape tuned to the two-pass search (whose scratch is large); an
+// instance's path sums wait for the tail of the previous instance on the
+// same handle.
 // Each instance's candidates land in its own pinned host area.
 // A gated batch (with a proposal network) is not captured: it runs as direct
-// launches on the same two streams (enqueue_gated).
+// launches on three streams (enqueue_gated).
 
 struct sc_batch {
   struct Instance {
@@ -1219,6 +1221,12 @@ struct sc_batch {
   hipStream_t front_stream = nullptr;
   std::vector<hipEvent_t> gate_events;
   std::vector<long> prev_on_handle;
+  // Gated batch: the network's input tiles, activations and logits for the
+  // blocks of one run (net_blocks: the most blocks of any run).
+  unsigned short* net_z = nullptr;
+  sc::pn::Acts net_a;
+  float* net_logits = nullptr;
+  long net_blocks = 0;
   // Kernel timing (plain batches): runs launch directly, with events around
   // each path-sum launch.
   bool timing = false;
@@ -1237,72 +1245,165 @@ void batch_release(sc_batch* b) {
   (void)hipHostFree(b->h_cand);
   (void)hipHostFree(b->h_stats);
   (void)hipHostFree(b->h_gate);
+  (void)hipFree(b->net_z);
+  (void)hipFree(b->net_a.h1);
+  (void)hipFree(b->net_a.h2);
+  (void)hipFree(b->net_a.h3);
+  (void)hipFree(b->net_logits);
   for (hipEvent_t ev : b->gate_events)
     if (ev) (void)hipEventDestroy(ev);
 }
 
+// The length of the run of consecutive instances from i that one launch of
+// the fused kernel searches (T_r = 16, the default shape): up to
+// kMaxFusedSet instances of the same shape and launch parameters, each on its
+// own handle; else 1.
+size_t set_run(const sc_batch* b, size_t i) {
+  const sc_batch::Instance& a = b->inst[i];
+  const sc_dd* da = b->shapes[a.shape];
+  if (da->rounded != 16 || da->use_two_pass ||
+      !sc::fused_set_supported(16, da->lanes, da->group, da->prefetch))
+    return 1;
+  size_t n = 1;
+  for (; n < (size_t)sc::kMaxFusedSet && i + n < b->inst.size(); ++n) {
+    const sc_batch::Instance& c = b->inst[i + n];
+    const sc_dd* dc = b->shapes[c.shape];
+    bool ok = da->rounded == dc->rounded && da->num_channels == dc->num_channels &&
+              da->foff == dc->foff && da->tsamp == dc->tsamp && a.in.dtype == c.in.dtype &&
+              a.in.row_stride == c.in.row_stride && a.r.bmin == c.r.bmin &&
+              a.r.bmax == c.r.bmax && da->lanes == dc->lanes && da->group == dc->group &&
+              da->prefetch == dc->prefetch;
+    for (size_t m = 0; ok && m < n; ++m) ok = b->inst[i + m].shape != c.shape;
+    if (!ok) break;
+  }
+  return n;
+}
+
+// The front of the run [a, e) of a gated batch, one launch per step for all
+// its channels: reset, column sums, statistics, pooling, the network over
+// all the run's blocks, the tile lists.
+sc_status enqueue_front_set(sc_batch* b, size_t a, size_t e, hipStream_t stream) {
+  sc::FrontSet fs = {};
+  for (size_t j = a; j < e; ++j) {
+    sc_dd* dj = b->shapes[b->inst[j].shape];
+    fs.x[j - a] = b->inst[j].in.data;
+    fs.key[j - a] = dj->top_key;
+    fs.hist[j - a] = dj->stats_hist;
+    fs.colsum[j - a] = dj->colsum;
+    fs.stats_scratch[j - a] = dj->stats_scratch;
+    fs.stats_out[j - a] = dj->stats_out;
+    fs.list[j - a] = dj->gate_list;
+    fs.count[j - a] = dj->gate_count;
+  }
+  const sc_batch::Instance& b0 = b->inst[a];
+  const sc_dd* d0 = b->shapes[b0.shape];
+  const int n = (int)(e - a), T = d0->rounded, N = d0->num_channels;
+  const int nblk = N / sc::pn::kBlockChannels;
+  if ((long)n * nblk > b->net_blocks) return fail(SC_ERR_INTERNAL, "gated batch: run too long");
+  SC_HIP(sc::launch_front_set_stats(fs, n, b0.in.dtype, b0.in.row_stride, T, N, stream),
+         "front statistics");
+  SC_HIP(sc::pn::launch_pool_set(fs, n, b0.in.dtype, b0.in.row_stride, nblk, b->net_z, stream),
+         "pool");
+  SC_HIP(sc::pn::launch_forward(b->pn->w, b->net_z, n * nblk, b->net_a, b->net_logits, nullptr,
+                                stream),
+         "proposal forward");
+  SC_HIP(sc::pn::launch_tile_list_set(b->net_logits, nblk, b->logit_threshold, N,
+                                      sc::fused_tile_width(T, d0->lanes), fs, n, stream),
+         "tile lists");
+  return SC_OK;
+}
+
 // Gated batch run, as direct launches on three streams joined by events
-// (device-side waits): the trees on one in instance order, each instance's
-// front (reset, column sums, statistics, pooling, network, tile list) on the
-// second, its tail (merge, offsets, copies) on the third. The front of
-// instance i + 1 is enqueued right after tree i, so it runs beside it, once
-// the tail of the previous instance on its handle is done. (Captured into a
-// graph, the same sequence ran each tree only after the previous instance's
-// tail; with the tails on the fronts' stream the fronts became the longer
-// chain.)
+// (device-side waits): the trees on one in instance order, the fronts (reset,
+// column sums, statistics, pooling, network, tile lists) on the second, the
+// tails (merge, offsets, copies) on the third. Runs of consecutive instances
+// of one shape (set_run) share one launch per step of the front and one tree
+// launch over their tile lists. The next run's front is enqueued right after
+// the current run's tree, so it runs beside it, once the tails of the
+// previous instances on its handles are done. (Captured into a graph, the
+// same sequence ran each tree only after the previous instance's tail; with
+// the tails on the fronts' stream the fronts became the longer chain. One
+// front per channel, eight of them took longer beside an eight-channel tree
+// than the tree itself.)
 sc_status enqueue_gated(sc_batch* b, hipStream_t stream) {
   const size_t n = b->inst.size();
   hipStream_t M = b->capture_stream, F = b->front_stream, Q = b->tail_stream;
   hipEvent_t* ev = b->gate_events.data();
-  auto front_ev = [&](size_t j) { return ev[2 + 3 * j]; };
-  auto sums_ev = [&](size_t j) { return ev[3 + 3 * j]; };
+  // Per run (indexed by its first instance): front and tree done; per
+  // instance: tail done.
+  auto front_ev = [&](size_t a) { return ev[2 + 3 * a]; };
+  auto sums_ev = [&](size_t a) { return ev[3 + 3 * a]; };
   auto tail_ev = [&](size_t j) { return ev[4 + 3 * j]; };
+  std::vector<std::pair<size_t, size_t>> runs;
+  for (size_t a = 0; a < n; a += runs.back().second - a) runs.push_back({a, a + set_run(b, a)});
   SC_HIP(hipEventRecord(ev[0], stream), "batch start");
   SC_HIP(hipStreamWaitEvent(M, ev[0], 0), "batch start");
   SC_HIP(hipStreamWaitEvent(F, ev[0], 0), "batch start");
   SC_HIP(hipStreamWaitEvent(Q, ev[0], 0), "batch start");
-  size_t next_front = 0;
-  // Fronts of the instances below `limit` whose handle is free once the
-  // tails of the instances below `tails` are enqueued.
-  auto issue_fronts = [&](long tails, size_t limit) -> sc_status {
-    for (; next_front < std::min(limit, n) && b->prev_on_handle[next_front] < tails;
-         ++next_front) {
-      sc_batch::Instance& bj = b->inst[next_front];
-      sc_dd* dj = b->shapes[bj.shape];
-      const long prev = b->prev_on_handle[next_front];
-      if (prev >= 0) SC_HIP(hipStreamWaitEvent(F, tail_ev((size_t)prev), 0), "wait tail");
-      SC_HIP(sc::launch_reset(dj->top_key, dj->num_channels, dj->stats_hist, F), "reset");
-      sc_status st = enqueue_gate_front(dj, b->pn, bj.in.data, bj.in.dtype, bj.in.row_stride,
-                                        b->logit_threshold, F);
-      if (st != SC_OK) return st;
-      SC_HIP(hipEventRecord(front_ev(next_front), F), "front done");
-    }
+  size_t fronts = 0;  // runs whose front is enqueued
+  // Enqueues the front of run r if the tails of the instances below `tails`
+  // cover the previous instance on each of its handles.
+  auto issue_front = [&](size_t r, long tails) -> sc_status {
+    if (r >= runs.size() || fronts != r) return SC_OK;
+    for (size_t j = runs[r].first; j < runs[r].second; ++j)
+      if (b->prev_on_handle[j] >= tails) return SC_OK;
+    for (size_t j = runs[r].first; j < runs[r].second; ++j)
+      if (b->prev_on_handle[j] >= 0)
+        SC_HIP(hipStreamWaitEvent(F, tail_ev((size_t)b->prev_on_handle[j]), 0), "wait tail");
+    sc_status st = enqueue_front_set(b, runs[r].first, runs[r].second, F);
+    if (st != SC_OK) return st;
+    SC_HIP(hipEventRecord(front_ev(runs[r].first), F), "front done");
+    ++fronts;
     return SC_OK;
   };
-  sc_status s = issue_fronts(0, 1);
-  for (size_t i = 0; i < n && s == SC_OK; ++i) {
-    sc_batch::Instance& bi = b->inst[i];
-    sc_dd* dd = b->shapes[bi.shape];
-    s = issue_fronts((long)i, i + 1);
+  sc_status s = SC_OK;
+  for (size_t r = 0; r < runs.size() && s == SC_OK; ++r) {
+    const size_t a = runs[r].first, e = runs[r].second;
+    // The run's instances are on distinct handles, so each one's previous
+    // instance on its handle comes before the run: the front is issued.
+    s = issue_front(r, (long)a);
     if (s != SC_OK) break;
-    SC_HIP(hipStreamWaitEvent(M, front_ev(i), 0), "wait front");
-    s = enqueue_gated_tree(dd, bi.in.data, bi.in.dtype, bi.in.row_stride, bi.r, M);
+    if (fronts <= r) return fail(SC_ERR_INTERNAL, "gated batch: front not issued");
+    SC_HIP(hipStreamWaitEvent(M, front_ev(a), 0), "wait front");
+    const sc_batch::Instance& b0 = b->inst[a];
+    sc_dd* d0 = b->shapes[b0.shape];
+    if (e - a == 1) {
+      s = enqueue_gated_tree(d0, b0.in.data, b0.in.dtype, b0.in.row_stride, b0.r, M);
+      if (s != SC_OK) break;
+    } else {
+      sc::FusedSet set = {};
+      for (size_t j = a; j < e; ++j) {
+        const sc_dd* dj = b->shapes[b->inst[j].shape];
+        set.x[j - a] = b->inst[j].in.data;
+        set.key[j - a] = dj->top_key;
+        set.list[j - a] = dj->gate_list;
+        set.count[j - a] = dj->gate_count;
+      }
+      SC_HIP(sc::launch_fused16_set(set, (int)(e - a), b0.in.dtype, b0.in.row_stride, 16,
+                                    d0->num_channels, b0.r.bmin, b0.r.bmax - b0.r.bmin + 1,
+                                    d0->lanes, d0->group, d0->prefetch, true, M, true),
+             "gated set tree");
+    }
+    SC_HIP(hipEventRecord(sums_ev(a), M), "tree done");
+    s = issue_front(r + 1, (long)a);
     if (s != SC_OK) break;
-    SC_HIP(hipEventRecord(sums_ev(i), M), "tree done");
-    s = issue_fronts((long)i, i + 2);
-    if (s != SC_OK) break;
-    SC_HIP(hipStreamWaitEvent(Q, sums_ev(i), 0), "wait tree");
-    s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, bi.in.data, bi.in.dtype,
-                            bi.in.row_stride, Q, b->h_cand + bi.cand_offset, b->h_stats + 4 * i,
-                            true, true);
-    if (s != SC_OK) break;
-    SC_HIP(hipMemcpyAsync(b->h_gate + 2 * i, dd->gate_count, 2 * sizeof(int),
-                          hipMemcpyDeviceToHost, Q),
-           "copy tile count");
-    SC_HIP(hipEventRecord(tail_ev(i), Q), "tail done");
+    SC_HIP(hipStreamWaitEvent(Q, sums_ev(a), 0), "wait tree");
+    for (size_t i = a; i < e; ++i) {
+      sc_batch::Instance& bi = b->inst[i];
+      sc_dd* dd = b->shapes[bi.shape];
+      s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, bi.in.data, bi.in.dtype,
+                              bi.in.row_stride, Q, b->h_cand + bi.cand_offset,
+                              b->h_stats + 4 * i, true, true);
+      if (s != SC_OK) break;
+      SC_HIP(hipMemcpyAsync(b->h_gate + 2 * i, dd->gate_count, 2 * sizeof(int),
+                            hipMemcpyDeviceToHost, Q),
+             "copy tile count");
+      SC_HIP(hipEventRecord(tail_ev(i), Q), "tail done");
+    }
   }
   if (s != SC_OK) return s;
-  // Tail i follows tree i and front i precedes it: the tails' stream ends last.
+  // Each run's tails follow its tree, and its front precedes it: the tails'
+  // stream ends last.
   SC_HIP(hipEventRecord(ev[1], Q), "batch done");
   SC_HIP(hipStreamWaitEvent(stream, ev[1], 0), "batch done");
   return SC_OK;
@@ -1330,35 +1431,8 @@ sc_status enqueue_plain(sc_batch* b, hipStream_t M, hipStream_t Q, NewEvent&& ne
   hipEvent_t tail_done = nullptr;
   // last_tail[h]: the tail event of the previous instance on handle h.
   std::vector<hipEvent_t> last_tail(b->shapes.size(), nullptr);
-  // Whether instance j can join a set launch with instance i: the same
-  // shape and launch parameters, on another handle.
-  auto same_launch = [&](const sc_batch::Instance& a, const sc_batch::Instance& c) {
-    const sc_dd* da = b->shapes[a.shape];
-    const sc_dd* dc = b->shapes[c.shape];
-    return a.shape != c.shape && da->rounded == dc->rounded &&
-           da->num_channels == dc->num_channels && da->foff == dc->foff &&
-           da->tsamp == dc->tsamp && a.in.dtype == c.in.dtype &&
-           a.in.row_stride == c.in.row_stride && a.r.bmin == c.r.bmin &&
-           a.r.bmax == c.r.bmax && da->lanes == dc->lanes && da->group == dc->group &&
-           da->prefetch == dc->prefetch;
-  };
   for (size_t i = 0; i < b->inst.size() && s == SC_OK && e == hipSuccess;) {
-    // A run of up to kMaxFusedSet consecutive instances searched by one
-    // launch of the fused kernel (T_r = 16, default shape), else one.
-    size_t n = 1;
-    {
-      const sc_dd* d0 = b->shapes[b->inst[i].shape];
-      if (d0->rounded == 16 && !d0->use_two_pass &&
-          sc::fused_set_supported(16, d0->lanes, d0->group, d0->prefetch)) {
-        while (n < (size_t)sc::kMaxFusedSet && i + n < b->inst.size()) {
-          bool ok = same_launch(b->inst[i], b->inst[i + n]);
-          for (size_t m = 1; ok && m < n; ++m)
-            ok = b->inst[i + m].shape != b->inst[i + n].shape;
-          if (!ok) break;
-          ++n;
-        }
-      }
-    }
+    const size_t n = set_run(b, i);
     for (size_t j = i; j < i + n && e == hipSuccess; ++j)
       if (last_tail[b->inst[j].shape])
         e = hipStreamWaitEvent(M, last_tail[b->inst[j].shape], 0);
@@ -1502,10 +1576,10 @@ sc_status batch_create(const sc_instance* instances, int count, const sc_tuning*
     }
     if (shape < 0) {
       shape = (int)b->shapes.size();
-      // Two handles per shape, used alternately; in a plain batch at 16 time
-      // steps two per channel of a set launch (runs of consecutive instances
-      // of the shape share one launch of the fused kernel, enqueue_plain).
-      int copies = T == 16 && !pn ? 2 * sc::kMaxFusedSet : 2;
+      // Two handles per shape, used alternately; at 16 time steps two per
+      // channel of a set launch (runs of consecutive instances of the shape
+      // share one launch of the fused kernel: enqueue_plain, enqueue_gated).
+      int copies = T == 16 ? 2 * sc::kMaxFusedSet : 2;
       for (int k = 0; k < num_tuning; ++k)
         if (tuning[k].rounded_timesteps == T && tuning[k].num_channels == in.num_channels &&
             T > 16 && tuning[k].pass_group != 0)
@@ -1585,6 +1659,32 @@ sc_status batch_create(const sc_instance* instances, int count, const sc_tuning*
         ev = nullptr;
         s = hip_fail(e, "event");
         break;
+      }
+    }
+    // The network's buffers for the blocks of the longest run (448 KiB per
+    // block: tiles and three activation maps, BF16).
+    for (size_t a = 0, len = 0; a < b->inst.size(); a += len) {
+      len = set_run(b, a);
+      const long blocks =
+          (long)len * (b->shapes[b->inst[a].shape]->num_channels / sc::pn::kBlockChannels);
+      b->net_blocks = std::max(b->net_blocks, blocks);
+    }
+    const size_t B = (size_t)b->net_blocks;
+    struct {
+      void** p;
+      size_t bytes;
+    } bufs[] = {{(void**)&b->net_z, B * sc::pn::kRows * sc::pn::kPooled * 2},
+                {(void**)&b->net_a.h1, B * sc::pn::kActElems1 * 2},
+                {(void**)&b->net_a.h2, B * sc::pn::kActElems2 * 2},
+                {(void**)&b->net_a.h3, B * sc::pn::kActElems3 * 2},
+                {(void**)&b->net_logits, B * sizeof(float)}};
+    for (auto& buf : bufs) {
+      if (s != SC_OK) break;
+      hipError_t e = hipMalloc(buf.p, buf.bytes);
+      if (e != hipSuccess) {
+        *buf.p = nullptr;
+        s = e == hipErrorOutOfMemory ? fail(SC_ERR_OUT_OF_MEMORY, "proposal network buffers")
+                                     : hip_fail(e, "proposal network buffers");
       }
     }
   }

@@ -114,8 +114,10 @@ __global__ void __launch_bounds__(kWave)
 // launcher makes one chunk per tile instead of about eight. The body below
 // the item decode is fused_tree_top_kernel's, repeated as text: written as one
 // shared inline function, the kernels compiled to ten more registers and the
-// single-channel one ran 2-3 % slower.
-template <int T, int C, int G, int MODE, typename In, bool BK>
+// single-channel one ran 2-3 % slower. LIST (gated batches): each channel's
+// items are its listed tiles' only (set.count[i] tiles x chunks), the active
+// items of all channels dealt over the XCDs; the rest of the grid exits.
+template <int T, int C, int G, int MODE, typename In, bool BK, bool LIST>
 __global__ void __launch_bounds__(kWave)
     fused_tree_top_set_kernel(FusedSet set, long stride, int N, int bmin, int nblocks,
                               int ngroups, int chunks, int groups_per_chunk,
@@ -125,15 +127,33 @@ __global__ void __launch_bounds__(kWave)
   __shared__ float win_storage[Wn::kFloats];
   lds_float* win = (lds_float*)win_storage;
 
-  const int per_xcd = gridDim.x / 8;
-  const int item_all = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  if (item_all >= total_items) return;
-  const int inst = item_all / items_per_channel;
-  const int item = item_all - inst * items_per_channel;
+  int inst = 0, item = 0;
+  if (LIST) {
+    const int n = total_items / items_per_channel;
+    int active = 0;
+    for (int i = 0; i < n; ++i) active += min(items_per_channel, *set.count[i] * chunks);
+    const int per_xcd = (active + 7) / 8;
+    if ((int)blockIdx.x / 8 >= per_xcd) return;
+    item = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (item >= active) return;
+    // item < the sum of the channels' active items: the walk stops at a channel below n.
+    for (;;) {
+      const int a = min(items_per_channel, *set.count[inst] * chunks);
+      if (item < a) break;
+      item -= a;
+      ++inst;
+    }
+  } else {
+    const int per_xcd = gridDim.x / 8;
+    const int item_all = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (item_all >= total_items) return;
+    inst = item_all / items_per_channel;
+    item = item_all - inst * items_per_channel;
+  }
   const In* __restrict__ x = (const In*)set.x[inst];
   unsigned long long* __restrict__ top_key = set.key[inst];
   float* __restrict__ colsum = set.colsum[inst];
-  const int tile = item / chunks;
+  const int tile = LIST ? set.list[inst][item / chunks] : item / chunks;
   const int chunk = item % chunks;
   const int c_w = tile * Sh::kOut;
   const int g_begin = chunk * groups_per_chunk;
@@ -253,7 +273,7 @@ hipError_t fused_dispatch(const In* x, long stride, int T, int N, int bmin, int 
 
 template <typename In>
 hipError_t launch_set_typed(const FusedSet& set, int n, long stride, int N, int bmin,
-                            int nblocks, bool block_keys, hipStream_t stream) {
+                            int nblocks, bool block_keys, bool lists, hipStream_t stream) {
   // The default 16-time-step shape (C = 4, G = 4, mode 2).
   constexpr int T = 16, C = 4, G = 4, MODE = 2;
   using Sh = TileShape<T, C>;
@@ -265,14 +285,24 @@ hipError_t launch_set_typed(const FusedSet& set, int n, long stride, int N, int 
   const long total = per_channel * n;
   if (total > (1L << 30)) return hipErrorInvalidValue;
   const int grid = (int)((total + 7) / 8 * 8);
-  if (block_keys)
-    hipLaunchKernelGGL((fused_tree_top_set_kernel<T, C, G, MODE, In, true>), dim3(grid),
+  if (lists) {
+    // Gated searches keep block keys only.
+    if (!block_keys) return hipErrorInvalidValue;
+    for (int i = 0; i < n; ++i)
+      if (!set.list[i] || !set.count[i]) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((fused_tree_top_set_kernel<T, C, G, MODE, In, true, true>), dim3(grid),
                        dim3(kWave), 0, stream, set, stride, N, bmin, nblocks, (int)ngroups,
                        (int)chunks, (int)per_chunk, (int)per_channel, (int)total);
-  else
-    hipLaunchKernelGGL((fused_tree_top_set_kernel<T, C, G, MODE, In, false>), dim3(grid),
+  } else if (block_keys) {
+    hipLaunchKernelGGL((fused_tree_top_set_kernel<T, C, G, MODE, In, true, false>), dim3(grid),
                        dim3(kWave), 0, stream, set, stride, N, bmin, nblocks, (int)ngroups,
                        (int)chunks, (int)per_chunk, (int)per_channel, (int)total);
+  } else {
+    hipLaunchKernelGGL((fused_tree_top_set_kernel<T, C, G, MODE, In, false, false>),
+                       dim3(grid), dim3(kWave), 0, stream, set, stride, N, bmin, nblocks,
+                       (int)ngroups, (int)chunks, (int)per_chunk, (int)per_channel,
+                       (int)total);
+  }
   return hipGetLastError();
 }
 
@@ -284,14 +314,17 @@ bool fused_set_supported(int T, int C, int G, int P) {
 
 hipError_t launch_fused16_set(const FusedSet& set, int n, int dtype, long stride, int T, int N,
                               int bmin, int nblocks, int C, int G, int P, bool block_keys,
-                              hipStream_t stream) {
+                              hipStream_t stream, bool lists) {
   if (n < 1 || n > kMaxFusedSet || !fused_set_supported(T, C, G, P)) return hipErrorInvalidValue;
   switch (dtype) {
-    case 0: return launch_set_typed<float>(set, n, stride, N, bmin, nblocks, block_keys, stream);
-    case 1: return launch_set_typed<__half>(set, n, stride, N, bmin, nblocks, block_keys, stream);
+    case 0:
+      return launch_set_typed<float>(set, n, stride, N, bmin, nblocks, block_keys, lists, stream);
+    case 1:
+      return launch_set_typed<__half>(set, n, stride, N, bmin, nblocks, block_keys, lists,
+                                      stream);
     case 2:
       return launch_set_typed<__hip_bfloat16>(set, n, stride, N, bmin, nblocks, block_keys,
-                                              stream);
+                                              lists, stream);
   }
   return hipErrorInvalidValue;
 }

@@ -67,13 +67,37 @@ struct FusedSet {
   const void* x[kMaxFusedSet];
   top_key_t* key[kMaxFusedSet];
   float* colsum[kMaxFusedSet];  // may be null (column sums not written)
+  // Gated set launches (launch_fused16_set with lists): per channel the
+  // ascending tile list and its length (device), as in launch_fused_tree_top.
+  const int* list[kMaxFusedSet];
+  const int* count[kMaxFusedSet];
 };
+// Per-channel buffers of a gated batch run's front: one launch per step for
+// all n channels (launch_front_set_stats, pn::launch_pool_set,
+// pn::launch_tile_list_set), grid.y = channel.
+struct FrontSet {
+  const void* x[kMaxFusedSet];
+  top_key_t* key[kMaxFusedSet];
+  unsigned* hist[kMaxFusedSet];
+  float* colsum[kMaxFusedSet];
+  void* stats_scratch[kMaxFusedSet];
+  float* stats_out[kMaxFusedSet];
+  int* list[kMaxFusedSet];
+  int* count[kMaxFusedSet];  // {tiles listed, blocks flagged}
+};
+// launch_reset, launch_colsum and launch_device_stats for n channels of one
+// shape, each step one launch for all of them.
+hipError_t launch_front_set_stats(const FrontSet& fs, int n, int dtype, long stride, int T,
+                                  int N, hipStream_t stream);
+
 // Whether launch_fused16_set has a kernel for (T, C, G, P) (the default
 // 16-time-step shape).
 bool fused_set_supported(int T, int C, int G, int P);
+// With lists (gated: block keys, only the listed tiles of each channel),
+// the grid is sized for every tile and the idle items exit.
 hipError_t launch_fused16_set(const FusedSet& set, int n, int dtype, long stride, int T, int N,
                               int bmin, int nblocks, int C, int G, int P, bool block_keys,
-                              hipStream_t stream);
+                              hipStream_t stream, bool lists = false);
 
 // The T_r <= 16 kernel of launch_fused_tree_top (dd_fused16.hip), same arguments.
 hipError_t launch_fused16(const void* x, int dtype, long stride, int T, int N, int bmin,

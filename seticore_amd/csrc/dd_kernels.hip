@@ -927,8 +927,8 @@ __global__ void top_update_kernel(const float* __restrict__ sums, int T, int N, 
 }
 
 template <typename In>
-__global__ void colsum_kernel(const In* __restrict__ x, long stride, int T, int N,
-                              float* __restrict__ colsum) {
+__device__ __forceinline__ void colsum_body(const In* __restrict__ x, long stride, int T, int N,
+                                            float* __restrict__ colsum) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
   float s = 0.0f;
@@ -936,16 +936,37 @@ __global__ void colsum_kernel(const In* __restrict__ x, long stride, int T, int 
   colsum[c] = s;
 }
 
+template <typename In>
+__global__ void colsum_kernel(const In* __restrict__ x, long stride, int T, int N,
+                              float* __restrict__ colsum) {
+  colsum_body<In>(x, stride, T, N, colsum);
+}
+
+// Column sums of several channels (grid.y = channel).
+template <typename In>
+__global__ void colsum_set_kernel(FrontSet fs, long stride, int T, int N) {
+  colsum_body<In>((const In*)fs.x[blockIdx.y], stride, T, N, fs.colsum[blockIdx.y]);
+}
+
 // Clears the top keys and the statistics histograms before a search (a fill
 // kernel rather than a memset, so the sequence can be captured in a graph).
-__global__ void reset_kernel(top_key_t* __restrict__ key, long n, unsigned* __restrict__ hist,
-                             long nh) {
+__device__ __forceinline__ void reset_body(top_key_t* __restrict__ key, long n,
+                                           unsigned* __restrict__ hist, long nh) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x)
     key[i] = 0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nh;
        i += (long)gridDim.x * blockDim.x)
     hist[i] = 0;
+}
+
+__global__ void reset_kernel(top_key_t* __restrict__ key, long n, unsigned* __restrict__ hist,
+                             long nh) {
+  reset_body(key, n, hist, nh);
+}
+
+__global__ void reset_set_kernel(FrontSet fs, long n, long nh) {
+  reset_body(fs.key[blockIdx.y], n, fs.hist[blockIdx.y], nh);
 }
 
 __global__ void unpack_top_kernel(const top_key_t* __restrict__ key, int N, int T, int bmin,
@@ -1111,9 +1132,11 @@ __device__ void block_histogram(const float* __restrict__ v, int N, unsigned* ld
   }
 }
 
-__global__ void __launch_bounds__(kStatsThreads)
-    stats_pass1_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
-  __shared__ unsigned h[kL1];
+// The statistics passes below are written as device functions (LDS passed
+// in) so that the single-channel kernels and the several-channel ones
+// (grid.y = channel, a gated batch run's front) share them.
+__device__ __forceinline__ void stats_pass1_body(const float* __restrict__ v, int N,
+                                                 unsigned* __restrict__ hist, unsigned* h) {
   for (int i = threadIdx.x; i < kL1; i += blockDim.x) h[i] = 0;
   __syncthreads();
   unsigned bad = 0;
@@ -1128,6 +1151,17 @@ __global__ void __launch_bounds__(kStatsThreads)
   __syncthreads();
   for (int i = threadIdx.x; i < kL1; i += blockDim.x)
     if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+__global__ void __launch_bounds__(kStatsThreads)
+    stats_pass1_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
+  __shared__ unsigned h[kL1];
+  stats_pass1_body(v, N, hist, h);
+}
+
+__global__ void __launch_bounds__(kStatsThreads) stats_pass1_set_kernel(FrontSet fs, int N) {
+  __shared__ unsigned h[kL1];
+  stats_pass1_body(fs.colsum[blockIdx.y], N, fs.hist[blockIdx.y], h);
 }
 
 // Level-1 selection (replicated in every workgroup).
@@ -1154,9 +1188,8 @@ __device__ void select_l2(const unsigned* __restrict__ hist, unsigned (&pre)[3],
   }
 }
 
-__global__ void __launch_bounds__(kStatsThreads)
-    stats_pass2_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
-  __shared__ unsigned h[3 * kL2 / 2];
+__device__ __forceinline__ void stats_pass2_body(const float* __restrict__ v, int N,
+                                                 unsigned* __restrict__ hist, unsigned* h) {
   unsigned pre[3], want[3], below[3];
   select_l1(hist, N, pre, want, below);
   const unsigned p0 = pre[0], p1 = pre[1], p2 = pre[2];
@@ -1170,8 +1203,18 @@ __global__ void __launch_bounds__(kStatsThreads)
 }
 
 __global__ void __launch_bounds__(kStatsThreads)
-    stats_pass3_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
-  __shared__ unsigned h[3 * kL3 / 2];
+    stats_pass2_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
+  __shared__ unsigned h[3 * kL2 / 2];
+  stats_pass2_body(v, N, hist, h);
+}
+
+__global__ void __launch_bounds__(kStatsThreads) stats_pass2_set_kernel(FrontSet fs, int N) {
+  __shared__ unsigned h[3 * kL2 / 2];
+  stats_pass2_body(fs.colsum[blockIdx.y], N, fs.hist[blockIdx.y], h);
+}
+
+__device__ __forceinline__ void stats_pass3_body(const float* __restrict__ v, int N,
+                                                 unsigned* __restrict__ hist, unsigned* h) {
   unsigned pre[3], want[3], below[3];
   select_l1(hist, N, pre, want, below);
   select_l2(hist, pre, want, below);
@@ -1183,6 +1226,17 @@ __global__ void __launch_bounds__(kStatsThreads)
         return top == (r == 0 ? p0 : (r == 1 ? p1 : p2));
       },
       [](unsigned k) { return k & (kL3 - 1); });
+}
+
+__global__ void __launch_bounds__(kStatsThreads)
+    stats_pass3_kernel(const float* __restrict__ v, int N, unsigned* __restrict__ hist) {
+  __shared__ unsigned h[3 * kL3 / 2];
+  stats_pass3_body(v, N, hist, h);
+}
+
+__global__ void __launch_bounds__(kStatsThreads) stats_pass3_set_kernel(FrontSet fs, int N) {
+  __shared__ unsigned h[3 * kL3 / 2];
+  stats_pass3_body(fs.colsum[blockIdx.y], N, fs.hist[blockIdx.y], h);
 }
 
 // All three levels: exact keys, counts below and equal.
@@ -1213,11 +1267,9 @@ __device__ __forceinline__ V block_sum(V x, V* sh) {
   return r;
 }
 
-__global__ void __launch_bounds__(kStatsThreads)
-    stats_pass4_kernel(const float* __restrict__ v, int N, const unsigned* __restrict__ hist,
-                       StatsState* st) {
-  __shared__ double shd[kStatsThreads];
-  __shared__ unsigned shu[kStatsThreads];
+__device__ __forceinline__ void stats_pass4_body(const float* __restrict__ v, int N,
+                                                 const unsigned* __restrict__ hist,
+                                                 StatsState* st, double* shd, unsigned* shu) {
   unsigned key[3], below[3], equal[3];
   select_all(hist, N, key, below, equal);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1252,9 +1304,23 @@ __global__ void __launch_bounds__(kStatsThreads)
 }
 
 __global__ void __launch_bounds__(kStatsThreads)
-    stats_final_kernel(const StatsState* st, int N, const unsigned* __restrict__ hist,
-                       float* __restrict__ stats_out) {
+    stats_pass4_kernel(const float* __restrict__ v, int N, const unsigned* __restrict__ hist,
+                       StatsState* st) {
   __shared__ double shd[kStatsThreads];
+  __shared__ unsigned shu[kStatsThreads];
+  stats_pass4_body(v, N, hist, st, shd, shu);
+}
+
+__global__ void __launch_bounds__(kStatsThreads) stats_pass4_set_kernel(FrontSet fs, int N) {
+  __shared__ double shd[kStatsThreads];
+  __shared__ unsigned shu[kStatsThreads];
+  stats_pass4_body(fs.colsum[blockIdx.y], N, fs.hist[blockIdx.y],
+                   (StatsState*)fs.stats_scratch[blockIdx.y], shd, shu);
+}
+
+__device__ __forceinline__ void stats_final_body(const StatsState* st, int N,
+                                                 const unsigned* __restrict__ hist,
+                                                 float* __restrict__ stats_out, double* shd) {
   const int tid = threadIdx.x;
   // Fixed-order tree reduction of the per-workgroup partials.
   const double s1 = block_sum<double>(st->part_s1[tid], shd);
@@ -1292,6 +1358,19 @@ __global__ void __launch_bounds__(kStatsThreads)
   stats_out[1] = (float)sqrt(fmax(acc, 0.0) / (double)cnt);
   stats_out[2] = m;
   stats_out[3] = (float)hist[kNonFiniteWord];  // non-finite column sums (0 for valid input)
+}
+
+__global__ void __launch_bounds__(kStatsThreads)
+    stats_final_kernel(const StatsState* st, int N, const unsigned* __restrict__ hist,
+                       float* __restrict__ stats_out) {
+  __shared__ double shd[kStatsThreads];
+  stats_final_body(st, N, hist, stats_out, shd);
+}
+
+__global__ void __launch_bounds__(kStatsThreads) stats_final_set_kernel(FrontSet fs, int N) {
+  __shared__ double shd[kStatsThreads];
+  stats_final_body((const StatsState*)fs.stats_scratch[blockIdx.y], N, fs.hist[blockIdx.y],
+                   fs.stats_out[blockIdx.y], shd);
 }
 
 // ---------------------------------------------------------------------------
@@ -1919,6 +1998,35 @@ hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsign
   hipLaunchKernelGGL(stats_pass3_kernel, grid, block, 0, stream, colsum, N, hist);
   hipLaunchKernelGGL(stats_pass4_kernel, grid, block, 0, stream, colsum, N, hist, st);
   hipLaunchKernelGGL(stats_final_kernel, dim3(1), block, 0, stream, st, N, hist, stats_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_front_set_stats(const FrontSet& fs, int n, int dtype, long stride, int T,
+                                  int N, hipStream_t stream) {
+  if (n < 1 || n > kMaxFusedSet) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(reset_set_kernel, dim3(blocks_for(N, 256, 2048), n), dim3(256), 0, stream,
+                     fs, (long)N, (long)stats_hist_words());
+  const dim3 cgrid((N + 255) / 256, n);
+  switch (dtype) {
+    case 0:
+      hipLaunchKernelGGL(colsum_set_kernel<float>, cgrid, dim3(256), 0, stream, fs, stride, T, N);
+      break;
+    case 1:
+      hipLaunchKernelGGL(colsum_set_kernel<__half>, cgrid, dim3(256), 0, stream, fs, stride, T,
+                         N);
+      break;
+    case 2:
+      hipLaunchKernelGGL(colsum_set_kernel<__hip_bfloat16>, cgrid, dim3(256), 0, stream, fs,
+                         stride, T, N);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  const dim3 grid(kStatsBlocks, n), block(kStatsThreads);
+  hipLaunchKernelGGL(stats_pass1_set_kernel, grid, block, 0, stream, fs, N);
+  hipLaunchKernelGGL(stats_pass2_set_kernel, grid, block, 0, stream, fs, N);
+  hipLaunchKernelGGL(stats_pass3_set_kernel, grid, block, 0, stream, fs, N);
+  hipLaunchKernelGGL(stats_pass4_set_kernel, grid, block, 0, stream, fs, N);
+  hipLaunchKernelGGL(stats_final_set_kernel, dim3(1, n), block, 0, stream, fs, N);
   return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 namespace sc {
+struct FrontSet;
 namespace pn {
 
 constexpr int kRows = 16;              // time steps of a tile (T_r of the search)
@@ -75,6 +76,15 @@ hipError_t launch_pack(Weights& w, hipStream_t stream);
 hipError_t launch_tile_list(const float* logits, int nblk, float threshold, int N,
                             int tile_width, int* list, int* count, int* flagged,
                             hipStream_t stream);
+
+// Several channels of one shape (a gated batch run's front, FrontSet in
+// dd_internal.h): launch_pool for each channel c into z + c * nblk tiles, and
+// launch_tile_list for each channel from logits + c * nblk into its list and
+// count (count[1] = flagged blocks); one launch each.
+hipError_t launch_pool_set(const FrontSet& fs, int n, int dtype, long stride, int nblk,
+                           unsigned short* z, hipStream_t stream);
+hipError_t launch_tile_list_set(const float* logits, int nblk, float threshold, int N,
+                                int tile_width, const FrontSet& fs, int n, hipStream_t stream);
 
 }  // namespace pn
 }  // namespace sc

@@ -30,6 +30,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "dd_internal.h"
 #include "pn_internal.h"
 
 namespace sc {
@@ -64,9 +65,9 @@ template <>
 __device__ __forceinline__ float load_f32<_Float16>(const _Float16* p) { return (float)*p; }
 
 template <typename In>
-__global__ void __launch_bounds__(256)
-    pool_kernel(const In* __restrict__ x, long stride, int nblk, const float* __restrict__ stats,
-                unsigned short* __restrict__ z) {
+__device__ __forceinline__ void pool_body(const In* __restrict__ x, long stride, int nblk,
+                                          const float* __restrict__ stats,
+                                          unsigned short* __restrict__ z) {
   const long i = blockIdx.x * 256L + threadIdx.x;  // over nblk * 16 * 1024
   if (i >= (long)nblk * kRows * kPooled) return;
   const int j = (int)(i % kPooled);
@@ -78,6 +79,22 @@ __global__ void __launch_bounds__(256)
   float v = 2.0f * (s - 0.25f * median) / sd;
   v = fminf(fmaxf(v, kZMin), kZMax);
   z[i] = f2bf(v);
+}
+
+template <typename In>
+__global__ void __launch_bounds__(256)
+    pool_kernel(const In* __restrict__ x, long stride, int nblk, const float* __restrict__ stats,
+                unsigned short* __restrict__ z) {
+  pool_body<In>(x, stride, nblk, stats, z);
+}
+
+// Several channels (grid.y = channel), channel c's tiles at z + c * nblk tiles.
+template <typename In>
+__global__ void __launch_bounds__(256)
+    pool_set_kernel(FrontSet fs, long stride, int nblk, unsigned short* __restrict__ z) {
+  const int c = blockIdx.y;
+  pool_body<In>((const In*)fs.x[c], stride, nblk, fs.stats_out[c],
+                z + (long)c * nblk * kRows * kPooled);
 }
 
 // ---------------------------------------------------------------------------
@@ -413,11 +430,10 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Gating: the channel tiles of the search that overlap a flagged block, in
 // ascending order (one workgroup; list[0..*count)).
-__global__ void __launch_bounds__(1024)
-    tile_list_kernel(const float* __restrict__ logits, int nblk, float threshold, int N,
-                     int tile_width, int* __restrict__ list, int* __restrict__ count,
-                     int* __restrict__ flagged) {
-  __shared__ int sums[1024];
+__device__ __forceinline__ void tile_list_body(const float* __restrict__ logits, int nblk,
+                                               float threshold, int N, int tile_width,
+                                               int* __restrict__ list, int* __restrict__ count,
+                                               int* __restrict__ flagged, int* sums) {
   const int ntiles = (N + tile_width - 1) / tile_width;
   const int per = (ntiles + 1023) / 1024;
   const int t0 = threadIdx.x * per;
@@ -456,6 +472,24 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+__global__ void __launch_bounds__(1024)
+    tile_list_kernel(const float* __restrict__ logits, int nblk, float threshold, int N,
+                     int tile_width, int* __restrict__ list, int* __restrict__ count,
+                     int* __restrict__ flagged) {
+  __shared__ int sums[1024];
+  tile_list_body(logits, nblk, threshold, N, tile_width, list, count, flagged, sums);
+}
+
+// Several channels (one workgroup each; channel c's logits at logits + c * nblk).
+__global__ void __launch_bounds__(1024)
+    tile_list_set_kernel(const float* __restrict__ logits, int nblk, float threshold, int N,
+                         int tile_width, FrontSet fs) {
+  __shared__ int sums[1024];
+  const int c = blockIdx.x;
+  tile_list_body(logits + (long)c * nblk, nblk, threshold, N, tile_width, fs.list[c],
+                 fs.count[c], fs.count[c] + 1, sums);
+}
+
 }  // namespace
 
 hipError_t launch_pool(const void* x, int dtype, long stride, int nblk, const float* stats,
@@ -474,6 +508,27 @@ hipError_t launch_pool(const void* x, int dtype, long stride, int nblk, const fl
     case 2:
       hipLaunchKernelGGL(pool_kernel<unsigned short>, grid, block, 0, stream,
                          (const unsigned short*)x, stride, nblk, stats, z);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pool_set(const FrontSet& fs, int n, int dtype, long stride, int nblk,
+                           unsigned short* z, hipStream_t stream) {
+  if (n < 1 || n > kMaxFusedSet) return hipErrorInvalidValue;
+  const long per = (long)nblk * kRows * kPooled;
+  const dim3 grid((unsigned)((per + 255) / 256), n), block(256);
+  switch (dtype) {
+    case 0:
+      hipLaunchKernelGGL(pool_set_kernel<float>, grid, block, 0, stream, fs, stride, nblk, z);
+      break;
+    case 1:
+      hipLaunchKernelGGL(pool_set_kernel<_Float16>, grid, block, 0, stream, fs, stride, nblk, z);
+      break;
+    case 2:
+      hipLaunchKernelGGL(pool_set_kernel<unsigned short>, grid, block, 0, stream, fs, stride,
+                         nblk, z);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -520,6 +575,14 @@ hipError_t launch_adam(Weights& w, float* m, float* v, const float* grad, float 
 hipError_t launch_pack(Weights& w, hipStream_t stream) {
   hipLaunchKernelGGL(pack_kernel, dim3((kC3 * 15 * kC2 + 255) / 256), dim3(256), 0, stream,
                      w.params, w.w1b, w.w2b, w.w3b);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_list_set(const float* logits, int nblk, float threshold, int N,
+                                int tile_width, const FrontSet& fs, int n, hipStream_t stream) {
+  if (n < 1 || n > kMaxFusedSet) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tile_list_set_kernel, dim3(n), dim3(1024), 0, stream, logits, nblk,
+                     threshold, N, tile_width, fs);
   return hipGetLastError();
 }
 
