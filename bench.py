@@ -694,9 +694,11 @@ def run_learned(args, torch, sc):
     for block recalls of 0.9, 0.99, 0.995 and 1.0, and the gated search is
     timed against the plain search on 64 further coarse channels (FP32 and
     BF16 storage), with hit recall versus the plain search - per call, and
-    batched (one graph for all coarse channels, the network of one beside the
-    tree of the previous one) against the plain batch. The headline value is
-    the combined form: gated, batched, BF16 storage, at 0.995 block recall."""
+    batched (runs of eight coarse channels: one launch per front step -
+    statistics, pooling, network, tile lists - and one tree launch over the
+    tile lists, the next run's front beside the current run's tree) against
+    the plain batch. The headline value is the combined form: gated, batched,
+    BF16 storage, at 0.995 block recall."""
     from seticore_amd import proposal as P
     snr = args.snr_threshold
     t0 = time.perf_counter()

@@ -416,10 +416,13 @@ sc_status sc_batch_copy_hits(const sc_batch* b, int instance, sc_hit* hits, size
 /* Gated batch: every instance searched as sc_dd_search_proposed does (16 time
  * steps, a multiple of 4096 channels, at most pn's max_batch blocks each).
  * Each run is direct launches on three streams (not a graph): the trees on
- * one, each instance's column sums, statistics, network and tile list on
- * another (beside the previous instance's tree), its merge on the third. pn must
- * outlive the batch; its weights are read at every run. sc_batch_info reports
- * 0 graph nodes for it. */
+ * one, the column sums, statistics, network and tile lists on another, the
+ * merges on the third. Runs of up to 8 consecutive instances of one shape
+ * share one launch per step (their network runs over all their blocks at
+ * once, in buffers the batch allocates: 448 KiB per block of the longest
+ * run) and one tree launch; a run's front runs beside the previous run's
+ * tree. pn must outlive the batch; its weights are read at every run.
+ * sc_batch_info reports 0 graph nodes for it. */
 sc_status sc_batch_create_proposed(const sc_instance* instances, int count,
                                    const sc_tuning* tuning, int num_tuning, sc_pn* pn,
                                    float logit_threshold, sc_batch** out);
