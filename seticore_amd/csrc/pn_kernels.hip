@@ -146,10 +146,10 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+  if constexpr (CIN == 1) {
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int pos = wo0 + 16 * m + r;
-    if constexpr (CIN == 1) {
+    for (int m = 0; m < 4; ++m) {
+      const int pos = wo0 + 16 * m + r;
       bf16x4 a;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -165,23 +165,39 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
       for (int n = 0; n < NT; ++n)
         acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[0][0][n], a, acc[m][n], 0, 0, 0);
-    } else {
+    }
+  } else {
+    // Every operand of the row first (32-bit offsets within the block's
+    // input; taps outside the input read offset 0 and are zeroed), then the
+    // products in the same (position tile, tap, K step, channel tile) order.
+    const unsigned short* blk_in = in + b * HI * WI * CIN;
+    bf16x4 a[4][15][KS];
 #pragma unroll
-      for (int t = 0; t < 15; ++t) {
-        const int hi = ho * SH + t / 5 - 1;
-        const int wi = pos * SW + t % 5 - 2;
-        const bool ok = hi >= 0 && hi < HI && wi >= 0 && wi < WI;
-        const unsigned short* src = in + ((b * HI + (ok ? hi : 0)) * WI + (ok ? wi : 0)) * CIN;
+    for (int t = 0; t < 15; ++t) {
+      const int hi = ho * SH + t / 5 - 1;
+      const bool row_ok = hi >= 0 && hi < HI;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int wi = (wo0 + 16 * m + r) * SW + t % 5 - 2;
+        const bool ok = row_ok && wi >= 0 && wi < WI;
+        const int off = ok ? (hi * WI + wi) * CIN + 4 * g : 0;
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-          bf16x4 a = *(const bf16x4*)(src + s * 16 + 4 * g);
-          if (!ok) a = bf16x4{0, 0, 0, 0};
-#pragma unroll
-          for (int n = 0; n < NT; ++n)
-            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[t][s][n], a, acc[m][n], 0, 0, 0);
+          const bf16x4 v = *(const bf16x4*)(blk_in + off + s * 16);
+          a[m][t][s] = ok ? v : bf16x4{0, 0, 0, 0};
         }
       }
     }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int t = 0; t < 15; ++t)
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+          for (int n = 0; n < NT; ++n)
+            acc[m][n] =
+                __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[t][s][n], a[m][t][s], acc[m][n], 0, 0, 0);
   }
 
 #pragma unroll
@@ -537,14 +553,17 @@ hipError_t launch_pool_set(const FrontSet& fs, int n, int dtype, long stride, in
 
 hipError_t launch_forward(const Weights& w, const unsigned short* z, int B, Acts& a,
                           float* logits, int* argmax, hipStream_t stream) {
-  // conv1's weights are one register tile: a row per workgroup; conv2 and
-  // conv3 (15 and 30 KB of weights per workgroup) take all their rows.
-  hipLaunchKernelGGL((conv_fwd_kernel<1, kC1, 1, 2, 1>), dim3(kW1 / 64, kH1, B), dim3(64), 0,
+  // Rows per workgroup: 4 / 4 / 2 (grid.y 4 / 2 / 2). Measured forward at
+  // 256 / 2048 blocks: all rows of conv2 and conv3 in one workgroup and a row
+  // per workgroup for conv1, 125 / 866 us; these, 108 / 865 us (the
+  // workgroups load their weights a few times more, the blocks' rows spread
+  // over more workgroups).
+  hipLaunchKernelGGL((conv_fwd_kernel<1, kC1, 1, 2, 4>), dim3(kW1 / 64, kH1 / 4, B), dim3(64), 0,
                      stream, z, kRows, kPooled, w.w1b, w.params + kOffB1, a.h1, kH1, kW1);
-  hipLaunchKernelGGL((conv_fwd_kernel<kC1, kC2, 2, 2, kH2>), dim3(kW2 / 64, 1, B), dim3(64), 0,
-                     stream, a.h1, kH1, kW1, w.w2b, w.params + kOffB2, a.h2, kH2, kW2);
-  hipLaunchKernelGGL((conv_fwd_kernel<kC2, kC3, 2, 2, kH3>), dim3(kW3 / 64, 1, B), dim3(64), 0,
-                     stream, a.h2, kH2, kW2, w.w3b, w.params + kOffB3, a.h3, kH3, kW3);
+  hipLaunchKernelGGL((conv_fwd_kernel<kC1, kC2, 2, 2, 4>), dim3(kW2 / 64, kH2 / 4, B), dim3(64),
+                     0, stream, a.h1, kH1, kW1, w.w2b, w.params + kOffB2, a.h2, kH2, kW2);
+  hipLaunchKernelGGL((conv_fwd_kernel<kC2, kC3, 2, 2, 2>), dim3(kW3 / 64, kH3 / 2, B), dim3(64),
+                     0, stream, a.h2, kH2, kW2, w.w3b, w.params + kOffB3, a.h3, kH3, kW3);
   hipLaunchKernelGGL(head_kernel, dim3(B), dim3(256), 0, stream, a.h3, w.params, logits, argmax);
   return hipGetLastError();
 }
