@@ -130,8 +130,10 @@ class Batch:
     many times.
 
     With `proposal` (a ProposalNet), every instance is searched as
-    Dedopplerer.search_proposed does, gated at `logit_threshold`; the network
-    of one instance runs beside the tree of the previous one."""
+    Dedopplerer.search_proposed does, gated at `logit_threshold`; runs of up
+    to eight consecutive instances of one shape share each front launch
+    (statistics, pooling, network, tile lists) and one tree launch, and the
+    next run's front runs beside the current run's tree."""
 
     def __init__(self, instances, tuning=(), proposal=None, logit_threshold=0.0):
         L = _lib()
