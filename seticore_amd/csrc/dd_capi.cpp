@@ -538,10 +538,10 @@ namespace {
 // column sums (dedoppler.cu:205-227).
 //
 // With overlap_stats (device statistics, one search at a time): the column
-// sums come first from their own kernel, and the statistics run on the
-// handle's second stream while the path-sum kernels run on `stream` (they
-// need only the column sums); `stream` waits for them at the end, so the
-// tail is then just the window merge.
+// sums (their own kernel) and the statistics run on the handle's second
+// stream while the path-sum kernels run on `stream` (the statistics need only
+// the column sums); `stream` waits for them at the end, so the tail is then
+// just the window merge.
 //
 // Searches that end in the device tail carry block keys when the one-pass
 // kernel is the T_r <= 16 or the staged one (block_key_search): the tail
@@ -562,9 +562,10 @@ sc_status enqueue_path_sums(sc_dd* dd, const void* data, int dtype, long row_str
   const bool one_pass = (T <= 16 || !dd->use_two_pass) &&
                         sc::fused_supported(T, dd->lanes, dd->group, dd->prefetch);
   if (overlap_stats) {
-    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, stream), "column sums");
     SC_HIP(hipEventRecord(dd->ev_fork, stream), "fork");
     SC_HIP(hipStreamWaitEvent(dd->aux_stream, dd->ev_fork, 0), "fork");
+    SC_HIP(sc::launch_colsum(data, dtype, row_stride, T, N, dd->colsum, dd->aux_stream),
+           "column sums");
     SC_HIP(sc::launch_device_stats(dd->colsum, N, dd->stats_scratch, dd->stats_hist,
                                    dd->stats_out, dd->aux_stream),
            "device stats");
