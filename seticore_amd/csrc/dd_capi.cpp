@@ -1437,8 +1437,40 @@ sc_status enqueue_plain(sc_batch* b, hipStream_t M, hipStream_t Q, NewEvent&& ne
   hipEvent_t tail_done = nullptr;
   // last_tail[h]: the tail event of the previous instance on handle h.
   std::vector<hipEvent_t> last_tail(b->shapes.size(), nullptr);
+  // The tails of run [i, i + n), after its path sums.
+  auto enqueue_tails = [&](size_t i, size_t n, hipEvent_t sums) {
+    e = hipStreamWaitEvent(Q, sums, 0);
+    for (size_t j = i; j < i + n && s == SC_OK && e == hipSuccess; ++j) {
+      sc_batch::Instance& bj = b->inst[j];
+      sc_dd* dj = b->shapes[bj.shape];
+      s = enqueue_device_tail(dj, bj.r, bj.in.snr_threshold, bj.in.data, bj.in.dtype,
+                              bj.in.row_stride, Q, b->h_cand + bj.cand_offset,
+                              b->h_stats + 4 * j);
+      if (s != SC_OK) break;
+      hipEvent_t tail = nullptr;
+      e = new_event(&tail);
+      if (e == hipSuccess) e = hipEventRecord(tail, Q);
+      last_tail[bj.shape] = tail;
+      tail_done = tail;
+    }
+  };
+  // A run's tails are enqueued after the next run's path sums (unless that
+  // run needs one of their handles): a captured graph submits its nodes in
+  // capture order, so the next tree then starts right after the previous
+  // one instead of after the previous run's tails were submitted.
+  size_t pend_i = 0, pend_n = 0;
+  hipEvent_t pend_sums = nullptr;
   for (size_t i = 0; i < b->inst.size() && s == SC_OK && e == hipSuccess;) {
     const size_t n = set_run(b, i);
+    bool shares = false;
+    for (size_t j = i; j < i + n; ++j)
+      for (size_t k = pend_i; k < pend_i + pend_n; ++k)
+        shares = shares || b->inst[j].shape == b->inst[k].shape;
+    if (pend_n && shares) {
+      enqueue_tails(pend_i, pend_n, pend_sums);
+      pend_n = 0;
+      if (s != SC_OK || e != hipSuccess) break;
+    }
     for (size_t j = i; j < i + n && e == hipSuccess; ++j)
       if (last_tail[b->inst[j].shape])
         e = hipStreamWaitEvent(M, last_tail[b->inst[j].shape], 0);
@@ -1482,23 +1514,14 @@ sc_status enqueue_plain(sc_batch* b, hipStream_t M, hipStream_t Q, NewEvent&& ne
     hipEvent_t sums = nullptr;
     e = new_event(&sums);
     if (e == hipSuccess) e = hipEventRecord(sums, M);
-    if (e == hipSuccess) e = hipStreamWaitEvent(Q, sums, 0);
     if (e != hipSuccess) break;
-    for (size_t j = i; j < i + n && s == SC_OK && e == hipSuccess; ++j) {
-      sc_batch::Instance& bj = b->inst[j];
-      sc_dd* dj = b->shapes[bj.shape];
-      s = enqueue_device_tail(dj, bj.r, bj.in.snr_threshold, bj.in.data, bj.in.dtype,
-                              bj.in.row_stride, Q, b->h_cand + bj.cand_offset,
-                              b->h_stats + 4 * j);
-      if (s != SC_OK) break;
-      hipEvent_t tail = nullptr;
-      e = new_event(&tail);
-      if (e == hipSuccess) e = hipEventRecord(tail, Q);
-      last_tail[bj.shape] = tail;
-      tail_done = tail;
-    }
+    if (pend_n) enqueue_tails(pend_i, pend_n, pend_sums);
+    pend_i = i;
+    pend_n = n;
+    pend_sums = sums;
     i += n;
   }
+  if (pend_n && s == SC_OK && e == hipSuccess) enqueue_tails(pend_i, pend_n, pend_sums);
   *tail_done_out = tail_done;
   if (e != hipSuccess && s == SC_OK) s = hip_fail(e, "batch launches");
   return s;
