@@ -102,7 +102,7 @@ struct sc_dd {
   void* stats_scratch = nullptr;
   unsigned* stats_hist = nullptr;  // level-1 and level-2 histograms
   float* stats_out = nullptr;  // {median, std, mean, count}
-  int4* cand = nullptr;        // per-window candidates
+  int4* cand = nullptr;        // per-window candidates (+ one slot: the statistics)
   size_t cand_capacity = 0;
   float* tree_a = nullptr;     // T_r*N scratch for the staged path and tree API
   float* tree_b = nullptr;
@@ -618,6 +618,9 @@ sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_thresho
                               const void* data, int dtype, long row_stride, hipStream_t stream,
                               int4* h_cand = nullptr, float* h_stats = nullptr,
                               bool stats_done = false, bool gated = false) {
+  // The handle's own buffers (a single search): the statistics ride in the
+  // candidates' copy, at h_cand[nwin] (one copy instead of two).
+  const bool own = !h_cand && !h_stats;
   if (!h_cand) h_cand = dd->h_cand;
   if (!h_stats) h_stats = dd->h_stats;
   const int T = dd->rounded, N = dd->num_channels;
@@ -628,17 +631,19 @@ sc_status enqueue_device_tail(sc_dd* dd, const DriftRange& r, double snr_thresho
   const bool bk = block_key_search(dd);
   SC_HIP(sc::launch_window_merge(dd->top_key, N, T, r.bmin, r.window, snr_threshold,
                                  dd->stats_out, dd->top_sum, dd->top_blk, dd->top_off, dd->cand,
-                                 bk, stream, gated),
+                                 bk, stream, gated, own),
          "window merge");
   const int nwin = (int)(((long)N + r.window - 1) / r.window);
   if (bk)
     SC_HIP(sc::launch_candidate_offsets(data, dtype, row_stride, T, N, nwin, dd->cand,
                                         dd->top_off, stream),
            "candidate offsets");
-  SC_HIP(hipMemcpyAsync(h_cand, dd->cand, nwin * sizeof(int4), hipMemcpyDeviceToHost, stream),
+  SC_HIP(hipMemcpyAsync(h_cand, dd->cand, (nwin + (own ? 1 : 0)) * sizeof(int4),
+                        hipMemcpyDeviceToHost, stream),
          "copy candidates");
-  SC_HIP(hipMemcpyAsync(h_stats, dd->stats_out, 4 * sizeof(float), hipMemcpyDeviceToHost,
-                        stream), "copy stats");
+  if (!own)
+    SC_HIP(hipMemcpyAsync(h_stats, dd->stats_out, 4 * sizeof(float), hipMemcpyDeviceToHost,
+                          stream), "copy stats");
   return SC_OK;
 }
 
@@ -699,6 +704,7 @@ sc_status device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int b
   if (!out) out = &dd->hits;
   if (num_timesteps <= 0) num_timesteps = dd->num_timesteps;
   const int N = dd->num_channels;
+  const int nwin = (int)(((long)N + r.window - 1) / r.window);
   if (h_stats[3] != 0.0f)
     return fail(SC_ERR_INVALID_ARGUMENT,
                 "input has NaN or infinite samples (" + std::to_string((long)h_stats[3]) +
@@ -706,7 +712,6 @@ sc_status device_hits(sc_dd* dd, const DriftRange& r, const sc_meta* meta, int b
   const float median = h_stats[0], std_dev = h_stats[1];
   dd->last_median = median;
   dd->last_std = std_dev;
-  const int nwin = (int)(((long)N + r.window - 1) / r.window);
   for (int i = 0; i < nwin; ++i) {
     const int4 c = h_cand[i];
     if (c.x < 0) continue;
@@ -891,7 +896,10 @@ sc_status sc_dd_search(sc_dd* dd, const void* data, int dtype, int rounded_times
       if (s != SC_OK) return s;
     }
     SC_HIP(hipStreamSynchronize(stream), "search");
-    s = device_hits(dd, r, meta, beam, coarse_channel, min_drift);
+    // The statistics came with the candidates (enqueue_device_tail).
+    const int nwin = (int)(((long)N + r.window - 1) / r.window);
+    s = device_hits(dd, r, meta, beam, coarse_channel, min_drift, nullptr,
+                    (const float*)(dd->h_cand + nwin));
     if (s != SC_OK) return s;
   }
   if (dd->timing) {

@@ -162,10 +162,13 @@ hipError_t launch_device_stats(const float* colsum, int N, void* scratch, unsign
 // the three top arrays. cand_out[i] = {index or -1, sum bits, drift block,
 // path offset} for window i. With block keys the offsets (unpacked and in
 // the candidates) are -1 until launch_candidate_offsets fills the candidates'.
+// With stats_slot, cand_out[number of windows] also receives the four
+// statistics (float bits), for one copy to the host.
 hipError_t launch_window_merge(const top_key_t* key, int N, int T, int bmin, int window,
                                double snr_threshold, const float* stats, float* top_sum,
                                int* top_blk, int* top_off, int4* cand_out, bool block_keys,
-                               hipStream_t stream, bool skip_empty = false);
+                               hipStream_t stream, bool skip_empty = false,
+                               bool stats_slot = false);
 // After a block-key merge (T_r <= 16): each candidate's path offset, from the
 // input of the search.
 hipError_t launch_candidate_offsets(const void* x, int dtype, long stride, int T, int N,

@@ -1392,6 +1392,11 @@ __global__ void __launch_bounds__(256)
   __shared__ int any_larger;
   const int wid = blockIdx.x;
   const int tid = threadIdx.x;
+  // flags & 4: the statistics also go to out[number of windows], so that one
+  // copy brings both to the host.
+  if ((flags & 4) && wid == 0 && tid == 0)
+    out[gridDim.x] = make_int4(__float_as_int(stats[0]), __float_as_int(stats[1]),
+                               __float_as_int(stats[2]), __float_as_int(stats[3]));
   const int lane = tid % kWave, wave = tid / kWave;
   const float median = stats[0];
   const float sd = stats[1];
@@ -2033,11 +2038,11 @@ hipError_t launch_front_set_stats(const FrontSet& fs, int n, int dtype, long str
 hipError_t launch_window_merge(const top_key_t* key, int N, int T, int bmin, int window,
                                double snr_threshold, const float* stats, float* top_sum,
                                int* top_blk, int* top_off, int4* cand_out, bool block_keys,
-                               hipStream_t stream, bool skip_empty) {
+                               hipStream_t stream, bool skip_empty, bool stats_slot) {
   const int nwin = (int)(((long)N + window - 1) / window);
   hipLaunchKernelGGL(window_merge_kernel, dim3(nwin), dim3(256), 0, stream, key, N, T, bmin,
                      window, snr_threshold, stats, top_sum, top_blk, top_off, cand_out,
-                     (block_keys ? 1 : 0) | (skip_empty ? 2 : 0));
+                     (block_keys ? 1 : 0) | (skip_empty ? 2 : 0) | (stats_slot ? 4 : 0));
   return hipGetLastError();
 }
 
