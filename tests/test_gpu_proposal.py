@@ -266,39 +266,41 @@ def test_trained_network_beats_the_brightest_pixel_heuristic(gpu):
     assert flags[1] < flags[0]
 
 
-def test_gated_batch_runs_share_a_tree_launch(gpu):
-    # Eleven BF16 coarse channels of one width: runs of eight and three share
-    # one launch of the fused kernel over their tile lists (each channel its
-    # own list, about half of its blocks flagged). Every instance equals its
-    # per-call gated search, hits and flagged blocks, over two runs in a row.
+@pytest.mark.parametrize("dtype", [sc.SC_BF16, sc.SC_F16])
+def test_gated_batch_runs_share_a_tree_launch(gpu, dtype):
+    # Eleven BF16 or FP16 coarse channels of one width: runs of eight and
+    # three share each front launch and one launch of the fused kernel over
+    # their tile lists (each channel its own list, about half of its blocks
+    # flagged). Every instance equals its per-call gated search, hits and
+    # flagged blocks, over two runs in a row.
     import torch
     from seticore_amd import batch as B
     N, n_cc = 1 << 17, 11
     nblk = N // 4096
     net = sc.ProposalNet(max_batch=nblk, seed=11)
     xs = [_coarse_channel(torch, N, 80 + i) for i in range(n_cc)]
-    data = [sc.normalize(x, sc.SC_BF16)[0] for x in xs]
+    data = [sc.normalize(x, dtype)[0] for x in xs]
     meta = sc.FilterbankMetadata(coarse_channel_size=N, foff=FOFF)
     dd = sc.Dedopplerer(T, N, FOFF, TSAMP)
     dd.set_stats_mode(sc.SC_STATS_DEVICE)
     logits = []
     for x in data:
-        dd.search(sc.FilterbankBuffer(T, N, data=x, dtype=sc.SC_BF16), meta, -1, 0, 10.0, 0.0,
+        dd.search(sc.FilterbankBuffer(T, N, data=x, dtype=dtype), meta, -1, 0, 10.0, 0.0,
                   10.0, [])
         st = dd.state()
         stats = torch.tensor([st["median"], st["std"]], dtype=torch.float32, device="cuda")
         logits.append(net.forward(net.pool(x, stats)).cpu())
-    with pytest.raises(ValueError):  # the tensor's sample type is BF16
+    with pytest.raises(ValueError):  # the tensor's sample type is not FP32
         net.pool(data[0], stats, sc.SC_F32)
     thr = float(torch.cat(logits).median())
     per_call, inst = [], []
     for i, x in enumerate(data):
         out = []
-        flagged = dd.search_proposed(sc.FilterbankBuffer(T, N, data=x, dtype=sc.SC_BF16), net,
+        flagged = dd.search_proposed(sc.FilterbankBuffer(T, N, data=x, dtype=dtype), net,
                                      meta, -1, i, 10.0, 0.0, 10.0, thr, out)
         per_call.append((flagged, out))
         inst.append(B.instance(x, T, FOFF, TSAMP, 10.0, 0.0, 10.0, coarse_channel=i,
-                               dtype=sc.SC_BF16, metadata=meta))
+                               dtype=dtype, metadata=meta))
     total = sum(f for f, _ in per_call)
     print("flagged %d of %d blocks, %d hits" % (total, n_cc * nblk,
                                                 sum(len(o) for _, o in per_call)))
