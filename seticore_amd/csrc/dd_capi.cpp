@@ -1229,9 +1229,10 @@ struct sc_batch {
   hipStream_t tail_stream = nullptr;
   hipGraphExec_t exec = nullptr;
   int nodes = 0;
-  // Gated batch: the fronts' stream; events {start, done, then front, tree
-  // and tail done per instance}; per instance the previous instance on its
-  // handle (-1 if none).
+  // Gated batch: the fronts' stream; events {start, done, then three per
+  // instance: front and tree done (used for a run's first instance) and
+  // tail done}; per instance the previous instance on its handle (-1 if
+  // none).
   hipStream_t front_stream = nullptr;
   std::vector<hipEvent_t> gate_events;
   std::vector<long> prev_on_handle;
