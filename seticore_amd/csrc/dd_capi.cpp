@@ -1446,15 +1446,27 @@ sc_status enqueue_plain(sc_batch* b, hipStream_t M, hipStream_t Q, NewEvent&& ne
   hipEvent_t tail_done = nullptr;
   // last_tail[h]: the tail event of the previous instance on handle h.
   std::vector<hipEvent_t> last_tail(b->shapes.size(), nullptr);
-  // The tails of run [i, i + n), after its path sums.
+  // The tails of run [i, i + n), after its path sums; a set run's statistics
+  // in one launch per pass for all its channels.
   auto enqueue_tails = [&](size_t i, size_t n, hipEvent_t sums) {
     e = hipStreamWaitEvent(Q, sums, 0);
+    if (e == hipSuccess && n > 1) {
+      sc::FrontSet fs = {};
+      for (size_t j = 0; j < n; ++j) {
+        sc_dd* dj = b->shapes[b->inst[i + j].shape];
+        fs.colsum[j] = dj->colsum;
+        fs.hist[j] = dj->stats_hist;
+        fs.stats_scratch[j] = dj->stats_scratch;
+        fs.stats_out[j] = dj->stats_out;
+      }
+      e = sc::launch_device_stats_set(fs, (int)n, b->shapes[b->inst[i].shape]->num_channels, Q);
+    }
     for (size_t j = i; j < i + n && s == SC_OK && e == hipSuccess; ++j) {
       sc_batch::Instance& bj = b->inst[j];
       sc_dd* dj = b->shapes[bj.shape];
       s = enqueue_device_tail(dj, bj.r, bj.in.snr_threshold, bj.in.data, bj.in.dtype,
                               bj.in.row_stride, Q, b->h_cand + bj.cand_offset,
-                              b->h_stats + 4 * j);
+                              b->h_stats + 4 * j, n > 1);
       if (s != SC_OK) break;
       hipEvent_t tail = nullptr;
       e = new_event(&tail);

@@ -89,6 +89,9 @@ struct FrontSet {
 // shape, each step one launch for all of them.
 hipError_t launch_front_set_stats(const FrontSet& fs, int n, int dtype, long stride, int T,
                                   int N, hipStream_t stream);
+// launch_device_stats for n channels (their colsum, hist, stats_scratch,
+// stats_out), each pass one launch for all of them.
+hipError_t launch_device_stats_set(const FrontSet& fs, int n, int N, hipStream_t stream);
 
 // Whether launch_fused16_set has a kernel for (T, C, G, P) (the default
 // 16-time-step shape).

@@ -2026,6 +2026,12 @@ hipError_t launch_front_set_stats(const FrontSet& fs, int n, int dtype, long str
       break;
     default: return hipErrorInvalidValue;
   }
+  const hipError_t e = hipGetLastError();
+  return e != hipSuccess ? e : launch_device_stats_set(fs, n, N, stream);
+}
+
+hipError_t launch_device_stats_set(const FrontSet& fs, int n, int N, hipStream_t stream) {
+  if (n < 1 || n > kMaxFusedSet) return hipErrorInvalidValue;
   const dim3 grid(kStatsBlocks, n), block(kStatsThreads);
   hipLaunchKernelGGL(stats_pass1_set_kernel, grid, block, 0, stream, fs, N);
   hipLaunchKernelGGL(stats_pass2_set_kernel, grid, block, 0, stream, fs, N);
