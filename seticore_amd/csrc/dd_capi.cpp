@@ -1447,7 +1447,8 @@ sc_status enqueue_plain(sc_batch* b, hipStream_t M, hipStream_t Q, NewEvent&& ne
   // last_tail[h]: the tail event of the previous instance on handle h.
   std::vector<hipEvent_t> last_tail(b->shapes.size(), nullptr);
   // The tails of run [i, i + n), after its path sums; a set run's statistics
-  // in one launch per pass for all its channels.
+  // (one launch per pass), window merges and candidate offsets in one launch
+  // each for all its channels.
   auto enqueue_tails = [&](size_t i, size_t n, hipEvent_t sums) {
     e = hipStreamWaitEvent(Q, sums, 0);
     if (e == hipSuccess && n > 1) {
@@ -1459,14 +1460,45 @@ sc_status enqueue_plain(sc_batch* b, hipStream_t M, hipStream_t Q, NewEvent&& ne
         fs.stats_scratch[j] = dj->stats_scratch;
         fs.stats_out[j] = dj->stats_out;
       }
-      e = sc::launch_device_stats_set(fs, (int)n, b->shapes[b->inst[i].shape]->num_channels, Q);
+      const sc_batch::Instance& b0 = b->inst[i];
+      const int N = b->shapes[b0.shape]->num_channels;
+      e = sc::launch_device_stats_set(fs, (int)n, N, Q);
+      // Window merges and candidate offsets (block keys), one launch each.
+      sc::TailSet ts = {};
+      for (size_t j = 0; j < n; ++j) {
+        const sc_batch::Instance& bj = b->inst[i + j];
+        sc_dd* dj = b->shapes[bj.shape];
+        ts.x[j] = bj.in.data;
+        ts.key[j] = dj->top_key;
+        ts.stats[j] = dj->stats_out;
+        ts.top_sum[j] = dj->top_sum;
+        ts.top_blk[j] = dj->top_blk;
+        ts.top_off[j] = dj->top_off;
+        ts.cand[j] = dj->cand;
+        ts.window[j] = bj.r.window;
+        ts.snr[j] = bj.in.snr_threshold;
+      }
+      if (e == hipSuccess) e = sc::launch_window_merge_set(ts, (int)n, N, 16, b0.r.bmin, Q);
+      if (e == hipSuccess)
+        e = sc::launch_candidate_offsets_set(ts, (int)n, b0.in.dtype, b0.in.row_stride, N, Q);
     }
     for (size_t j = i; j < i + n && s == SC_OK && e == hipSuccess; ++j) {
       sc_batch::Instance& bj = b->inst[j];
       sc_dd* dj = b->shapes[bj.shape];
-      s = enqueue_device_tail(dj, bj.r, bj.in.snr_threshold, bj.in.data, bj.in.dtype,
-                              bj.in.row_stride, Q, b->h_cand + bj.cand_offset,
-                              b->h_stats + 4 * j, n > 1);
+      if (n > 1) {
+        // The candidates and statistics to the host (merged above).
+        const int nwin = (int)(((long)dj->num_channels + bj.r.window - 1) / bj.r.window);
+        e = hipMemcpyAsync(b->h_cand + bj.cand_offset, dj->cand, nwin * sizeof(int4),
+                           hipMemcpyDeviceToHost, Q);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(b->h_stats + 4 * j, dj->stats_out, 4 * sizeof(float),
+                             hipMemcpyDeviceToHost, Q);
+        if (e != hipSuccess) break;
+      } else {
+        s = enqueue_device_tail(dj, bj.r, bj.in.snr_threshold, bj.in.data, bj.in.dtype,
+                                bj.in.row_stride, Q, b->h_cand + bj.cand_offset,
+                                b->h_stats + 4 * j);
+      }
       if (s != SC_OK) break;
       hipEvent_t tail = nullptr;
       e = new_event(&tail);

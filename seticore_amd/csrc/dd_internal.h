@@ -93,6 +93,26 @@ hipError_t launch_front_set_stats(const FrontSet& fs, int n, int dtype, long str
 // stats_out), each pass one launch for all of them.
 hipError_t launch_device_stats_set(const FrontSet& fs, int n, int N, hipStream_t stream);
 
+// Per-channel buffers and parameters of a batch run's tail (16 time steps,
+// block keys): launch_window_merge and launch_candidate_offsets for n
+// channels of one width and drift-block range, one launch each (grid.y =
+// channel).
+struct TailSet {
+  const void* x[kMaxFusedSet];
+  const top_key_t* key[kMaxFusedSet];
+  const float* stats[kMaxFusedSet];
+  float* top_sum[kMaxFusedSet];
+  int* top_blk[kMaxFusedSet];
+  int* top_off[kMaxFusedSet];
+  int4* cand[kMaxFusedSet];
+  int window[kMaxFusedSet];
+  double snr[kMaxFusedSet];
+};
+hipError_t launch_window_merge_set(const TailSet& ts, int n, int N, int T, int bmin,
+                                   hipStream_t stream);
+hipError_t launch_candidate_offsets_set(const TailSet& ts, int n, int dtype, long stride, int N,
+                                        hipStream_t stream);
+
 // Whether launch_fused16_set has a kernel for (T, C, G, P) (the default
 // 16-time-step shape).
 bool fused_set_supported(int T, int C, int G, int P);

@@ -96,9 +96,9 @@ __device__ float path_sum_exact(const In* __restrict__ x, long stride, int N, in
 // unpacked offset of that channel. min(T, 64) lanes per window, one offset
 // each per round.
 template <int T, typename In>
-__global__ void __launch_bounds__(kWave)
-    candidate_offsets_kernel(const In* __restrict__ x, long stride, int N, int nwin,
-                             int4* __restrict__ cand, int* __restrict__ top_off) {
+__device__ __forceinline__ void candidate_offsets_body(const In* __restrict__ x, long stride,
+                                                       int N, int nwin, int4* __restrict__ cand,
+                                                       int* __restrict__ top_off) {
   constexpr int kLanes = T < kWave ? T : kWave;
   constexpr int kPerWave = kWave / kLanes;
   const int slot = threadIdx.x / kLanes;
@@ -135,6 +135,22 @@ __global__ void __launch_bounds__(kWave)
     cand[w] = r;
     top_off[r.x] = r.w;
   }
+}
+
+template <int T, typename In>
+__global__ void __launch_bounds__(kWave)
+    candidate_offsets_kernel(const In* __restrict__ x, long stride, int N, int nwin,
+                             int4* __restrict__ cand, int* __restrict__ top_off) {
+  candidate_offsets_body<T, In>(x, stride, N, nwin, cand, top_off);
+}
+
+// Several channels (grid.y = channel; each checks its own window count).
+template <int T, typename In>
+__global__ void __launch_bounds__(kWave) candidate_offsets_set_kernel(TailSet ts, long stride,
+                                                                      int N) {
+  const int c = blockIdx.y;
+  const int nwin = (int)(((long)N + ts.window[c] - 1) / ts.window[c]);
+  candidate_offsets_body<T, In>((const In*)ts.x[c], stride, N, nwin, ts.cand[c], ts.top_off[c]);
 }
 
 // Tree of one drift block for each T-row time block (T <= 16), written out in
@@ -1378,11 +1394,13 @@ __global__ void __launch_bounds__(kStatsThreads) stats_final_set_kernel(FrontSet
 // unpacking of the keys into the reference's three top arrays (each
 // workgroup unpacks its own window; channels without a positive path keep
 // their previous block and offset, dedoppler.cu:205).
-__global__ void __launch_bounds__(256)
-    window_merge_kernel(const top_key_t* __restrict__ key, int N, int T, int bmin, int window,
-                        double snr_threshold, const float* __restrict__ stats,
-                        float* __restrict__ top_sum, int* __restrict__ top_blk,
-                        int* __restrict__ top_off, int4* __restrict__ out, int flags) {
+__device__ __forceinline__ void window_merge_body(const top_key_t* __restrict__ key, int N, int T,
+                                                  int bmin, int window, double snr_threshold,
+                                                  const float* __restrict__ stats,
+                                                  float* __restrict__ top_sum,
+                                                  int* __restrict__ top_blk,
+                                                  int* __restrict__ top_off,
+                                                  int4* __restrict__ out, int flags) {
   const bool block_keys = flags & 1;
   // Gated search: a channel without a key (not searched, or no positive
   // path) neither forms nor suppresses a candidate.
@@ -1470,6 +1488,26 @@ __global__ void __launch_bounds__(256)
       out[wid] = make_int4(bi, __float_as_int(bv), top_blk[bi], top_off[bi]);
     }
   }
+}
+
+__global__ void __launch_bounds__(256)
+    window_merge_kernel(const top_key_t* __restrict__ key, int N, int T, int bmin, int window,
+                        double snr_threshold, const float* __restrict__ stats,
+                        float* __restrict__ top_sum, int* __restrict__ top_blk,
+                        int* __restrict__ top_off, int4* __restrict__ out, int flags) {
+  window_merge_body(key, N, T, bmin, window, snr_threshold, stats, top_sum, top_blk, top_off, out,
+                    flags);
+}
+
+// Several channels' merges (grid.y = channel, grid.x = the most windows of
+// any of them).
+__global__ void __launch_bounds__(256)
+    window_merge_set_kernel(TailSet ts, int N, int T, int bmin, int flags) {
+  const int c = blockIdx.y;
+  const int nwin = (int)(((long)N + ts.window[c] - 1) / ts.window[c]);
+  if ((int)blockIdx.x >= nwin) return;
+  window_merge_body(ts.key[c], N, T, bmin, ts.window[c], ts.snr[c], ts.stats[c], ts.top_sum[c],
+                    ts.top_blk[c], ts.top_off[c], ts.cand[c], flags);
 }
 
 __global__ void gather_kernel(const float* __restrict__ sums, int N, const int* __restrict__ offs,
@@ -1873,6 +1911,47 @@ hipError_t candidate_offsets_dispatch(const In* x, long stride, int T, int N, in
       return candidate_offsets_typed<256, In>(x, stride, N, nwin, cand, top_off, stream);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_window_merge_set(const TailSet& ts, int n, int N, int T, int bmin,
+                                   hipStream_t stream) {
+  if (n < 1 || n > kMaxFusedSet) return hipErrorInvalidValue;
+  int nwin = 0;
+  for (int c = 0; c < n; ++c) {
+    if (ts.window[c] <= 0) return hipErrorInvalidValue;
+    nwin = std::max(nwin, (int)(((long)N + ts.window[c] - 1) / ts.window[c]));
+  }
+  hipLaunchKernelGGL(window_merge_set_kernel, dim3(nwin, n), dim3(256), 0, stream, ts, N, T,
+                     bmin, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_candidate_offsets_set(const TailSet& ts, int n, int dtype, long stride, int N,
+                                        hipStream_t stream) {
+  if (n < 1 || n > kMaxFusedSet) return hipErrorInvalidValue;
+  int nwin = 0;
+  for (int c = 0; c < n; ++c) {
+    if (ts.window[c] <= 0) return hipErrorInvalidValue;
+    nwin = std::max(nwin, (int)(((long)N + ts.window[c] - 1) / ts.window[c]));
+  }
+  constexpr int per_wave = kWave / 16;
+  const dim3 grid((nwin + per_wave - 1) / per_wave, n);
+  switch (dtype) {
+    case 0:
+      hipLaunchKernelGGL((candidate_offsets_set_kernel<16, float>), grid, dim3(kWave), 0, stream,
+                         ts, stride, N);
+      break;
+    case 1:
+      hipLaunchKernelGGL((candidate_offsets_set_kernel<16, __half>), grid, dim3(kWave), 0, stream,
+                         ts, stride, N);
+      break;
+    case 2:
+      hipLaunchKernelGGL((candidate_offsets_set_kernel<16, __hip_bfloat16>), grid, dim3(kWave), 0,
+                         stream, ts, stride, N);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_candidate_offsets(const void* x, int dtype, long stride, int T, int N,
