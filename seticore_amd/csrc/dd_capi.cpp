@@ -1403,12 +1403,46 @@ sc_status enqueue_gated(sc_batch* b, hipStream_t stream) {
     s = issue_front(r + 1, (long)a);
     if (s != SC_OK) break;
     SC_HIP(hipStreamWaitEvent(Q, sums_ev(a), 0), "wait tree");
+    if (e - a > 1) {
+      // The run's window merges (channels without a key skipped) and
+      // candidate offsets, one launch each.
+      sc::TailSet ts = {};
+      for (size_t j = a; j < e; ++j) {
+        const sc_batch::Instance& bj = b->inst[j];
+        sc_dd* dj = b->shapes[bj.shape];
+        ts.x[j - a] = bj.in.data;
+        ts.key[j - a] = dj->top_key;
+        ts.stats[j - a] = dj->stats_out;
+        ts.top_sum[j - a] = dj->top_sum;
+        ts.top_blk[j - a] = dj->top_blk;
+        ts.top_off[j - a] = dj->top_off;
+        ts.cand[j - a] = dj->cand;
+        ts.window[j - a] = bj.r.window;
+        ts.snr[j - a] = bj.in.snr_threshold;
+      }
+      SC_HIP(sc::launch_window_merge_set(ts, (int)(e - a), d0->num_channels, 16, b0.r.bmin, Q,
+                                         true),
+             "window merges");
+      SC_HIP(sc::launch_candidate_offsets_set(ts, (int)(e - a), b0.in.dtype, b0.in.row_stride,
+                                              d0->num_channels, Q),
+             "candidate offsets");
+    }
     for (size_t i = a; i < e; ++i) {
       sc_batch::Instance& bi = b->inst[i];
       sc_dd* dd = b->shapes[bi.shape];
-      s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, bi.in.data, bi.in.dtype,
-                              bi.in.row_stride, Q, b->h_cand + bi.cand_offset,
-                              b->h_stats + 4 * i, true, true);
+      if (e - a > 1) {
+        const int nwin = (int)(((long)dd->num_channels + bi.r.window - 1) / bi.r.window);
+        SC_HIP(hipMemcpyAsync(b->h_cand + bi.cand_offset, dd->cand, nwin * sizeof(int4),
+                              hipMemcpyDeviceToHost, Q),
+               "copy candidates");
+        SC_HIP(hipMemcpyAsync(b->h_stats + 4 * i, dd->stats_out, 4 * sizeof(float),
+                              hipMemcpyDeviceToHost, Q),
+               "copy stats");
+      } else {
+        s = enqueue_device_tail(dd, bi.r, bi.in.snr_threshold, bi.in.data, bi.in.dtype,
+                                bi.in.row_stride, Q, b->h_cand + bi.cand_offset,
+                                b->h_stats + 4 * i, true, true);
+      }
       if (s != SC_OK) break;
       SC_HIP(hipMemcpyAsync(b->h_gate + 2 * i, dd->gate_count, 2 * sizeof(int),
                             hipMemcpyDeviceToHost, Q),

@@ -109,7 +109,7 @@ struct TailSet {
   double snr[kMaxFusedSet];
 };
 hipError_t launch_window_merge_set(const TailSet& ts, int n, int N, int T, int bmin,
-                                   hipStream_t stream);
+                                   hipStream_t stream, bool skip_empty = false);
 hipError_t launch_candidate_offsets_set(const TailSet& ts, int n, int dtype, long stride, int N,
                                         hipStream_t stream);
 

@@ -1914,7 +1914,7 @@ hipError_t candidate_offsets_dispatch(const In* x, long stride, int T, int N, in
 }
 
 hipError_t launch_window_merge_set(const TailSet& ts, int n, int N, int T, int bmin,
-                                   hipStream_t stream) {
+                                   hipStream_t stream, bool skip_empty) {
   if (n < 1 || n > kMaxFusedSet) return hipErrorInvalidValue;
   int nwin = 0;
   for (int c = 0; c < n; ++c) {
@@ -1922,7 +1922,7 @@ hipError_t launch_window_merge_set(const TailSet& ts, int n, int N, int T, int b
     nwin = std::max(nwin, (int)(((long)N + ts.window[c] - 1) / ts.window[c]));
   }
   hipLaunchKernelGGL(window_merge_set_kernel, dim3(nwin, n), dim3(256), 0, stream, ts, N, T,
-                     bmin, 1);
+                     bmin, 1 | (skip_empty ? 2 : 0));
   return hipGetLastError();
 }
 
